@@ -1,0 +1,62 @@
+"""Builds libgrpp_amd.so (HIP kernels for gfx950 + C API) in-tree with hipcc / gcc.
+
+    python -m libgrpp_amd.build
+
+Objects go to <repo>/_build/, the library to libgrpp_amd/libgrpp_amd.so.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INC = os.path.join(ROOT, "include")
+BUILD = os.path.join(ROOT, "_build")
+LIB = os.path.join(PKG, "libgrpp_amd.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("GRPP_OFFLOAD_ARCH", "gfx950")
+# packed FP32 instructions are disabled for all device code
+HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l and "not a recognized feature" not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f"build step failed: {' '.join(cmd)}\n{out}")
+    return out
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    inc = [f"-I{INC}", f"-I{CSRC}"]
+    objs = []
+    steps = [
+        ([HIPCC] + HIP_FLAGS + inc + ["-c", os.path.join(CSRC, "engine.hip"), "-o", os.path.join(BUILD, "engine.o")],
+         os.path.join(BUILD, "engine.o")),
+        (["g++", "-O3", "-fPIC", "-std=c++17"] + inc + ["-c", os.path.join(CSRC, "tables.cpp"), "-o",
+                                                         os.path.join(BUILD, "tables.o")], os.path.join(BUILD, "tables.o")),
+        (["gcc", "-O3", "-fPIC", "-std=gnu11", "-Wall"] + inc + ["-c", os.path.join(CSRC, "api.c"), "-o",
+                                                                 os.path.join(BUILD, "api.o")], os.path.join(BUILD, "api.o")),
+        (["gcc", "-O3", "-fPIC", "-std=gnu11", "-Wall"] + inc + ["-c", os.path.join(CSRC, "api_batch.c"), "-o",
+                                                                 os.path.join(BUILD, "api_batch.o")],
+         os.path.join(BUILD, "api_batch.o")),
+    ]
+    for cmd, obj in steps:
+        out = _run(cmd)
+        if verbose and out.strip():
+            print(out)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", "-o", tmp] + objs + ["-lm"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

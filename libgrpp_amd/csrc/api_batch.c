@@ -1,0 +1,553 @@
+/*
+ * libgrpp_amd C API, part 2: gradients (grpp_integrals_gradient.c) and the batched entry points.
+ *
+ * Gradients follow the reference's construction: d/dA of a Cartesian Gaussian is
+ * -n <L-1| + 2a <L+1| (diff_gaussian.c:35-63), so the engine evaluates full-GRPP blocks of the
+ * L-1 / L+1 shells and the host scatters them with the reference's geometry cases
+ * (grpp_integrals_gradient.c:280-393):
+ *     A != C :  D = A gets +<dA|U_C|B>,  D = C gets -<dA|U_C|B>
+ *     B != C :  D = B gets +<A|U_C|dB>,  D = C gets -<A|U_C|dB>
+ * which reproduces the ACB / ACA / ACC / CCB branches (translational invariance).
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "engine.h"
+#include "libgrpp.h"
+
+static void fail(const char *who)
+{
+    fprintf(stderr, "libgrpp_amd: %s: %s\n", who, libgrpp_last_error());
+    abort();
+}
+
+static int points_equal(const double *a, const double *b)
+{
+    const double t = 1e-12;   /* utils.c:96-107 */
+    return fabs(a[0] - b[0]) < t && fabs(a[1] - b[1]) < t && fabs(a[2] - b[2]) < t;
+}
+
+static int cidx_of(int n, int l, int m)
+{
+    int L = n + l + m;
+    return (L - n) * (L - n + 1) / 2 + (L - n - l);
+}
+
+static double nfac(double a, int L) { return pow(2 * a / M_PI, 0.75) * pow(4 * a, 0.5 * L); }
+
+/* L-1 and L+1 shells with the norm-factor ratios absorbed (diff_gaussian.c:35-63) */
+static void diff_shell(const libgrpp_shell_t *s, libgrpp_shell_t **down, libgrpp_shell_t **up)
+{
+    double *c = (double *) malloc(sizeof(double) * (s->num_primitives + 1));
+    double origin[3] = {s->origin[0], s->origin[1], s->origin[2]};
+    *down = NULL;
+    if (s->L > 0) {
+        for (int i = 0; i < s->num_primitives; i++) c[i] = s->coeffs[i] * nfac(s->alpha[i], s->L) / nfac(s->alpha[i], s->L - 1);
+        *down = libgrpp_new_shell(origin, s->L - 1, s->num_primitives, c, s->alpha);
+    }
+    for (int i = 0; i < s->num_primitives; i++)
+        c[i] = s->coeffs[i] * 2.0 * s->alpha[i] * nfac(s->alpha[i], s->L) / nfac(s->alpha[i], s->L + 1);
+    *up = libgrpp_new_shell(origin, s->L + 1, s->num_primitives, c, s->alpha);
+    free(c);
+}
+
+/*
+ * bra-derivative blocks: out[op][k][a][b] = -n_k(a) Dn[op][a-e_k][b] + Up[op][a+e_k][b]
+ * (canonical Cartesian order; Dn may be NULL for L = 0).  Dn/Up are op-major blocks
+ * with op stride sdn / sup and row length nB.
+ */
+static void bra_derivative(int LA, int nB, const double *Dn, long long sdn, const double *Up, long long sup,
+                           double *out /* [4][3][ncA*nB] */)
+{
+    int ncA = (LA + 1) * (LA + 2) / 2;
+    for (int a = 0; a < ncA; a++) {
+        int nl[3];
+        /* canonical component a */
+        int r = LA, base = 0;
+        while (a >= base + (LA - r + 1)) {
+            base += LA - r + 1;
+            r--;
+        }
+        nl[0] = r;
+        nl[1] = LA - r - (a - base);
+        nl[2] = LA - nl[0] - nl[1];
+        for (int k = 0; k < 3; k++) {
+            int up[3] = {nl[0], nl[1], nl[2]};
+            up[k] += 1;
+            int iu = cidx_of(up[0], up[1], up[2]);
+            int id = -1;
+            if (nl[k] > 0 && Dn) {
+                int dn[3] = {nl[0], nl[1], nl[2]};
+                dn[k] -= 1;
+                id = cidx_of(dn[0], dn[1], dn[2]);
+            }
+            for (int op = 0; op < 4; op++)
+                for (int b = 0; b < nB; b++) {
+                    double v = Up[op * sup + (long long) iu * nB + b];
+                    if (id >= 0) v -= nl[k] * Dn[op * sdn + (long long) id * nB + b];
+                    out[((op * 3 + k) * ncA + a) * nB + b] = v;
+                }
+        }
+    }
+}
+
+/* ---------------------------------------------------------------- per-call gradients */
+static void gradient_task(const char *who, libgrpp_shell_t *A, libgrpp_shell_t *B, const eng_center_t *cen,
+                          const double *C, const double *D, int parts, double **ga, double **gx, double **gy,
+                          double **gz)
+{
+    if (!libgrpp_is_initialized()) {
+        eng_set_error("libgrpp_init() has not been called");
+        fail(who);
+    }
+    int nA = A->cart_size, nB = B->cart_size, sz = nA * nB;
+    double **outs[4] = {ga, gx, gy, gz};
+    for (int op = 0; op < 4; op++)
+        for (int k = 0; k < 3; k++)
+            if (outs[op]) memset(outs[op][k], 0, sizeof(double) * sz);
+    int AC = points_equal(A->origin, C), BC = points_equal(B->origin, C);
+    if (AC && BC) return;
+    if (!points_equal(A->origin, D) && !points_equal(B->origin, D) && !points_equal(C, D)) return;
+
+    libgrpp_shell_t *Ad, *Au, *Bd, *Bu;
+    diff_shell(A, &Ad, &Au);
+    diff_shell(B, &Bd, &Bu);
+    libgrpp_shell_t *sh[6] = {A, B, Au, Bu, Ad, Bd};
+    int ns = 4 + (Ad != NULL) + (Bd != NULL);
+    int iAd = Ad ? 4 : -1, iBd = Bd ? (Ad ? 5 : 4) : -1;
+    if (!Ad && Bd) sh[4] = Bd;
+    /* blocks: (Au,B) (Ad,B) (A,Bu) (A,Bd), op-major inside one buffer each */
+    int ncAu = Au->cart_size, ncAd = Ad ? Ad->cart_size : 0, ncBu = Bu->cart_size, ncBd = Bd ? Bd->cart_size : 0;
+    long long o0 = 0, o1 = (long long) ncAu * nB, o2 = o1 + (long long) ncAd * nB, o3 = o2 + (long long) nA * ncBu;
+    long long tot = o3 + (long long) nA * ncBd;
+    eng_item_t it[4];
+    int nit = 0;
+    it[nit++] = (eng_item_t){2, 1, 0, 1, o0, 0, nB, 0};
+    if (Ad) it[nit++] = (eng_item_t){iAd, 1, 0, 1, o1, 0, nB, 0};
+    it[nit++] = (eng_item_t){0, 3, 0, 1, o2, 0, ncBu, 0};
+    if (Bd) it[nit++] = (eng_item_t){0, iBd, 0, 1, o3, 0, ncBd, 0};
+    int cidx = 0;
+    double *buf = (double *) malloc(sizeof(double) * 4 * (tot > 0 ? tot : 1));
+    if (eng_run(ns, sh, 1, cen, 1, &cidx, nit, it, tot, tot, parts, buf)) fail(who);
+
+    double *bra = (double *) calloc(12 * sz, sizeof(double));
+    double *ket = (double *) calloc(12 * sz, sizeof(double));
+    bra_derivative(A->L, nB, Ad ? buf + o1 : NULL, tot, buf + o0, tot, bra);
+    /* ket derivative: transpose B's blocks into (B', A) form, differentiate as a bra, transpose back */
+    {
+        double *tu = (double *) malloc(sizeof(double) * 4 * ncBu * nA);
+        double *td = Bd ? (double *) malloc(sizeof(double) * 4 * ncBd * nA) : NULL;
+        for (int op = 0; op < 4; op++) {
+            for (int i = 0; i < nA; i++)
+                for (int j = 0; j < ncBu; j++) tu[op * ncBu * nA + j * nA + i] = buf[op * tot + o2 + i * ncBu + j];
+            if (Bd)
+                for (int i = 0; i < nA; i++)
+                    for (int j = 0; j < ncBd; j++) td[op * ncBd * nA + j * nA + i] = buf[op * tot + o3 + i * ncBd + j];
+        }
+        double *kt = (double *) malloc(sizeof(double) * 12 * sz);
+        bra_derivative(B->L, nA, td, (long long) ncBd * nA, tu, (long long) ncBu * nA, kt);
+        for (int op = 0; op < 4; op++)
+            for (int k = 0; k < 3; k++)
+                for (int j = 0; j < nB; j++)
+                    for (int i = 0; i < nA; i++) ket[((op * 3 + k) * nA + i) * nB + j] = kt[((op * 3 + k) * nB + j) * nA + i];
+        free(tu);
+        free(td);
+        free(kt);
+    }
+    int AD = points_equal(A->origin, D), BD = points_equal(B->origin, D), CD = points_equal(C, D);
+    for (int op = 0; op < 4; op++) {
+        if (!outs[op]) continue;
+        for (int k = 0; k < 3; k++) {
+            double *g = outs[op][k];
+            const double *br = bra + (op * 3 + k) * sz, *kt = ket + (op * 3 + k) * sz;
+            if (!AC) {
+                if (AD) for (int e = 0; e < sz; e++) g[e] += br[e];
+                if (CD) for (int e = 0; e < sz; e++) g[e] -= br[e];
+            }
+            if (!BC) {
+                if (BD) for (int e = 0; e < sz; e++) g[e] += kt[e];
+                if (CD) for (int e = 0; e < sz; e++) g[e] -= kt[e];
+            }
+        }
+    }
+    /* caller's Cartesian order */
+    for (int op = 0; op < 4; op++) {
+        if (!outs[op]) continue;
+        for (int k = 0; k < 3; k++) {
+            double *g = outs[op][k];
+            double *tmp = (double *) malloc(sizeof(double) * sz);
+            memcpy(tmp, g, sizeof(double) * sz);
+            for (int i = 0; i < nA; i++) {
+                int ci = cidx_of(A->cart_list[3 * i], A->cart_list[3 * i + 1], A->cart_list[3 * i + 2]);
+                for (int j = 0; j < nB; j++) {
+                    int cj = cidx_of(B->cart_list[3 * j], B->cart_list[3 * j + 1], B->cart_list[3 * j + 2]);
+                    g[i * nB + j] = tmp[ci * nB + cj];
+                }
+            }
+            free(tmp);
+        }
+    }
+    free(bra);
+    free(ket);
+    free(buf);
+    libgrpp_delete_shell(Au);
+    libgrpp_delete_shell(Bu);
+    if (Ad) libgrpp_delete_shell(Ad);
+    if (Bd) libgrpp_delete_shell(Bd);
+}
+
+void libgrpp_full_grpp_integrals_gradient(libgrpp_shell_t *A, libgrpp_shell_t *B, libgrpp_grpp_t *g, double *C,
+                                          double *D, double **ga, double **gx, double **gy, double **gz)
+{
+    eng_center_t c;
+    memset(&c, 0, sizeof(c));
+    c.origin = C;
+    c.local = g->U_L;
+    c.n_arep = g->n_arep;
+    c.arep = g->U_arep;
+    c.n_so = g->n_esop;
+    c.so = g->U_esop;
+    c.n_oc = g->n_oc_shells;
+    c.oc_pot = g->U_oc;
+    c.oc_shell = g->oc_shells;
+    c.shell_end = -1;
+    gradient_task("libgrpp_full_grpp_integrals_gradient", A, B, &c, C, D, ENG_ALL, ga, gx, gy, gz);
+}
+
+void libgrpp_type1_integrals_gradient(libgrpp_shell_t *A, libgrpp_shell_t *B, double *C, libgrpp_potential_t *pot,
+                                      double *D, double **ga)
+{
+    eng_center_t c;
+    memset(&c, 0, sizeof(c));
+    c.origin = C;
+    c.local = pot;
+    c.shell_end = -1;
+    gradient_task("libgrpp_type1_integrals_gradient", A, B, &c, C, D, ENG_TYPE1, ga, NULL, NULL, NULL);
+}
+
+void libgrpp_type2_integrals_gradient(libgrpp_shell_t *A, libgrpp_shell_t *B, double *C, libgrpp_potential_t *pot,
+                                      double *D, double **ga)
+{
+    eng_center_t c;
+    memset(&c, 0, sizeof(c));
+    c.origin = C;
+    c.n_arep = 1;
+    c.arep = &pot;
+    c.shell_end = -1;
+    gradient_task("libgrpp_type2_integrals_gradient", A, B, &c, C, D, ENG_TYPE2, ga, NULL, NULL, NULL);
+}
+
+void libgrpp_spin_orbit_integrals_gradient(libgrpp_shell_t *A, libgrpp_shell_t *B, double *C, libgrpp_potential_t *pot,
+                                           double *D, double **gx, double **gy, double **gz)
+{
+    /* without the 2/(2L+1) factor (grpp_integrals_gradient.c:229-237) */
+    const double one = 1.0;
+    eng_center_t c;
+    memset(&c, 0, sizeof(c));
+    c.origin = C;
+    c.n_so = 1;
+    c.so = &pot;
+    c.so_scale = &one;
+    c.shell_end = -1;
+    gradient_task("libgrpp_spin_orbit_integrals_gradient", A, B, &c, C, D, ENG_SO, NULL, gx, gy, gz);
+}
+
+void libgrpp_outercore_potential_integrals_gradient(libgrpp_shell_t *A, libgrpp_shell_t *B, double *C, int n_oc,
+                                                    libgrpp_potential_t **pots, libgrpp_shell_t **shells, double *D,
+                                                    double **ga, double **gx, double **gy, double **gz)
+{
+    eng_center_t c;
+    memset(&c, 0, sizeof(c));
+    c.origin = C;
+    c.n_oc = n_oc;
+    c.oc_pot = pots;
+    c.oc_shell = shells;
+    c.shell_end = -1;
+    gradient_task("libgrpp_outercore_potential_integrals_gradient", A, B, &c, C, D, ENG_OC, ga, gx, gy, gz);
+}
+
+/* ---------------------------------------------------------------- batched entry points */
+struct libgrpp_batch_plan {
+    eng_plan_t *eng;
+    int dim;
+    int flags;
+};
+
+static int check_canonical(int n, libgrpp_shell_t *const *shells)
+{
+    for (int i = 0; i < n; i++)
+        for (int c = 0; c < shells[i]->cart_size; c++) {
+            const int *t = shells[i]->cart_list + 3 * c;
+            if (cidx_of(t[0], t[1], t[2]) != c) {
+                eng_set_error("shell %d: batched entry points need the standard Cartesian order", i);
+                return 0;
+            }
+        }
+    return 1;
+}
+
+static eng_center_t *make_centers(int ncenters, const double *xyz, libgrpp_grpp_t *const *g, int *cidx, int *nidx)
+{
+    eng_center_t *cs = (eng_center_t *) calloc(ncenters > 0 ? ncenters : 1, sizeof(eng_center_t));
+    *nidx = 0;
+    for (int c = 0; c < ncenters; c++) {
+        cs[c].origin = xyz + 3 * c;
+        cs[c].shell_end = -1;
+        if (!g[c]) continue;
+        cs[c].local = g[c]->U_L;
+        cs[c].n_arep = g[c]->n_arep;
+        cs[c].arep = g[c]->U_arep;
+        cs[c].n_so = g[c]->n_esop;
+        cs[c].so = g[c]->U_esop;
+        cs[c].n_oc = g[c]->n_oc_shells;
+        cs[c].oc_pot = g[c]->U_oc;
+        cs[c].oc_shell = g[c]->oc_shells;
+        cidx[(*nidx)++] = c;
+    }
+    return cs;
+}
+
+int libgrpp_batch_plan_create(int nshells, libgrpp_shell_t *const *shells, int ncenters, const double *xyz,
+                              libgrpp_grpp_t *const *grpps, int flags, libgrpp_batch_plan_t **plan)
+{
+    *plan = NULL;
+    if (!check_canonical(nshells, shells)) return 2;
+    int *off = (int *) malloc(sizeof(int) * (nshells + 1));
+    off[0] = 0;
+    for (int i = 0; i < nshells; i++) off[i + 1] = off[i] + shells[i]->cart_size;
+    int dim = off[nshells];
+    int *cidx = (int *) malloc(sizeof(int) * (ncenters > 0 ? ncenters : 1));
+    int nidx = 0;
+    eng_center_t *cs = make_centers(ncenters, xyz, grpps, cidx, &nidx);
+    int sym = !(flags & LIBGRPP_BATCH_NO_SYMMETRY);
+    int nit = sym ? nshells * (nshells + 1) / 2 : nshells * nshells;
+    eng_item_t *it = (eng_item_t *) malloc(sizeof(eng_item_t) * (nit > 0 ? nit : 1));
+    int k = 0;
+    for (int i = 0; i < nshells; i++)
+        for (int j = sym ? i : 0; j < nshells; j++) {
+            eng_item_t e;
+            e.a = i;
+            e.b = j;
+            e.cb = 0;
+            e.ce = nidx;
+            e.off = (long long) off[i] * dim + off[j];
+            e.moff = (long long) off[j] * dim + off[i];
+            e.stride = dim;
+            e.mirror = sym && (i != j);
+            it[k++] = e;
+        }
+    eng_plan_t *eng = NULL;
+    long long n2 = (long long) dim * dim;
+    int rc = eng_plan_create(nshells, shells, ncenters, cs, nidx, cidx, nit, it, n2, n2, ENG_ALL,
+                             (flags & LIBGRPP_BATCH_LEARNED_GRID) ? ENG_F_LEARNED : 0, &eng);
+    free(off);
+    free(cidx);
+    free(cs);
+    free(it);
+    if (rc) return rc;
+    libgrpp_batch_plan_t *p = (libgrpp_batch_plan_t *) calloc(1, sizeof(*p));
+    p->eng = eng;
+    p->dim = dim;
+    p->flags = flags;
+    *plan = p;
+    return 0;
+}
+
+int libgrpp_batch_plan_execute(libgrpp_batch_plan_t *p, void *stream) { return eng_plan_execute(p->eng, stream); }
+
+int libgrpp_batch_plan_download(libgrpp_batch_plan_t *p, double *arep, double *so_x, double *so_y, double *so_z)
+{
+    long long n2 = (long long) p->dim * p->dim;
+    double *buf = (double *) malloc(sizeof(double) * 4 * (n2 > 0 ? n2 : 1));
+    int rc = eng_plan_download(p->eng, buf);
+    if (!rc) {
+        memcpy(arep, buf, sizeof(double) * n2);
+        memcpy(so_x, buf + n2, sizeof(double) * n2);
+        memcpy(so_y, buf + 2 * n2, sizeof(double) * n2);
+        memcpy(so_z, buf + 3 * n2, sizeof(double) * n2);
+    }
+    free(buf);
+    return rc;
+}
+
+int libgrpp_batch_plan_dim(const libgrpp_batch_plan_t *p) { return p->dim; }
+int libgrpp_batch_plan_stats(const libgrpp_batch_plan_t *p, libgrpp_batch_stats_t *s) { return eng_plan_stats(p->eng, s); }
+int libgrpp_batch_plan_features(libgrpp_batch_plan_t *p, int *n, float *f) { return eng_plan_features(p->eng, n, f); }
+int libgrpp_batch_plan_set_predictions(libgrpp_batch_plan_t *p, int n, const float *v)
+{
+    return eng_plan_set_predictions(p->eng, n, v);
+}
+
+void libgrpp_batch_plan_destroy(libgrpp_batch_plan_t *p)
+{
+    if (!p) return;
+    eng_plan_destroy(p->eng);
+    free(p);
+}
+
+int libgrpp_batch_full_grpp_integrals(int nshells, libgrpp_shell_t *const *shells, int ncenters, const double *xyz,
+                                      libgrpp_grpp_t *const *grpps, int flags, double *arep, double *so_x,
+                                      double *so_y, double *so_z, libgrpp_batch_stats_t *stats)
+{
+    libgrpp_batch_plan_t *p = NULL;
+    int rc = libgrpp_batch_plan_create(nshells, shells, ncenters, xyz, grpps, flags, &p);
+    if (rc) return rc;
+    rc = libgrpp_batch_plan_execute(p, NULL);
+    if (!rc) rc = libgrpp_batch_plan_download(p, arep, so_x, so_y, so_z);
+    if (!rc && stats) libgrpp_batch_plan_stats(p, stats);
+    libgrpp_batch_plan_destroy(p);
+    return rc;
+}
+
+/*
+ * Batched gradients: the engine evaluates, per centre, the blocks <i-|U_C|j> and <i+|U_C|j> for
+ * every ordered pair; ket derivatives reuse them through <i|U|j+-> = s <j+-|U|i>^T
+ * (s = +1 arep, -1 spin-orbit).
+ */
+int libgrpp_batch_full_grpp_gradients(int nshells, libgrpp_shell_t *const *shells, int ncenters, const double *xyz,
+                                      libgrpp_grpp_t *const *grpps, int flags, double *out,
+                                      libgrpp_batch_stats_t *stats)
+{
+    (void) flags;
+    if (!check_canonical(nshells, shells)) return 2;
+    int *off = (int *) malloc(sizeof(int) * (nshells + 1));
+    off[0] = 0;
+    for (int i = 0; i < nshells; i++) off[i + 1] = off[i] + shells[i]->cart_size;
+    int dim = off[nshells];
+    long long n2 = (long long) dim * dim;
+    memset(out, 0, sizeof(double) * 12 * ncenters * n2);
+
+    /* extended shell list: basis, then (down, up) per basis shell */
+    int next = nshells;
+    libgrpp_shell_t **sh = (libgrpp_shell_t **) malloc(sizeof(void *) * 3 * (nshells + 1));
+    int *idn = (int *) malloc(sizeof(int) * (nshells + 1)), *iup = (int *) malloc(sizeof(int) * (nshells + 1));
+    for (int i = 0; i < nshells; i++) sh[i] = shells[i];
+    for (int i = 0; i < nshells; i++) {
+        libgrpp_shell_t *d, *u;
+        diff_shell(shells[i], &d, &u);
+        idn[i] = d ? next : -1;
+        if (d) sh[next++] = d;
+        iup[i] = next;
+        sh[next++] = u;
+    }
+    int *cidx = (int *) malloc(sizeof(int) * (ncenters > 0 ? ncenters : 1));
+    int nidx = 0;
+    eng_center_t *cs = make_centers(ncenters, xyz, grpps, cidx, &nidx);
+    /* items: per (i, j, centre) the down and up blocks, op-major in one buffer */
+    long long nitem_max = 2LL * nshells * nshells * (nidx > 0 ? nidx : 1);
+    eng_item_t *it = (eng_item_t *) malloc(sizeof(eng_item_t) * nitem_max);
+    long long *bup = (long long *) malloc(sizeof(long long) * nshells * nshells * (nidx > 0 ? nidx : 1));
+    long long *bdn = (long long *) malloc(sizeof(long long) * nshells * nshells * (nidx > 0 ? nidx : 1));
+    long long tot = 0;
+    int nit = 0;
+    for (int i = 0; i < nshells; i++)
+        for (int j = 0; j < nshells; j++)
+            for (int ci = 0; ci < nidx; ci++) {
+                long long key = ((long long) i * nshells + j) * nidx + ci;
+                int ncb = shells[j]->cart_size;
+                bdn[key] = -1;
+                if (idn[i] >= 0) {
+                    it[nit++] = (eng_item_t){idn[i], j, ci, ci + 1, tot, 0, ncb, 0};
+                    bdn[key] = tot;
+                    tot += (long long) sh[idn[i]]->cart_size * ncb;
+                }
+                it[nit++] = (eng_item_t){iup[i], j, ci, ci + 1, tot, 0, ncb, 0};
+                bup[key] = tot;
+                tot += (long long) sh[iup[i]]->cart_size * ncb;
+            }
+    double *buf = (double *) malloc(sizeof(double) * 4 * (tot > 0 ? tot : 1));
+    eng_plan_t *eng = NULL;
+    int rc = eng_plan_create(next, sh, ncenters, cs, nidx, cidx, nit, it, tot, tot, ENG_ALL, 0, &eng);
+    if (!rc) rc = eng_plan_execute(eng, NULL);
+    if (!rc) rc = eng_plan_download(eng, buf);
+    if (!rc && stats) eng_plan_stats(eng, stats);
+    eng_plan_destroy(eng);
+
+    /* atom of each shell: the centre at its position */
+    int *atom = (int *) malloc(sizeof(int) * (nshells + 1));
+    for (int i = 0; i < nshells; i++) {
+        atom[i] = -1;
+        for (int c = 0; c < ncenters; c++)
+            if (points_equal(shells[i]->origin, xyz + 3 * c)) { atom[i] = c; break; }
+    }
+    if (!rc) {
+        int maxnc = 0;
+        for (int i = 0; i < nshells; i++) if (shells[i]->cart_size > maxnc) maxnc = shells[i]->cart_size;
+        double *bra = (double *) malloc(sizeof(double) * 12 * maxnc * maxnc);
+        double *ket = (double *) malloc(sizeof(double) * 12 * maxnc * maxnc);
+        double *tu = (double *) malloc(sizeof(double) * 4 * (maxnc + 20) * maxnc * 2);
+        double *td = (double *) malloc(sizeof(double) * 4 * (maxnc + 20) * maxnc * 2);
+        for (int i = 0; i < nshells; i++)
+            for (int j = 0; j < nshells; j++)
+                for (int ci = 0; ci < nidx; ci++) {
+                    int c = cidx[ci];
+                    const double *C = xyz + 3 * c;
+                    int AC = points_equal(shells[i]->origin, C), BC = points_equal(shells[j]->origin, C);
+                    if (AC && BC) continue;
+                    int nA = shells[i]->cart_size, nB = shells[j]->cart_size;
+                    long long kij = ((long long) i * nshells + j) * nidx + ci, kji = ((long long) j * nshells + i) * nidx + ci;
+                    if (!AC)
+                        bra_derivative(shells[i]->L, nB, bdn[kij] >= 0 ? buf + bdn[kij] : NULL, tot, buf + bup[kij], tot, bra);
+                    if (!BC) {
+                        /* <i|U|j+-> = s <j+-|U|i>^T, then differentiate as a bra over j */
+                        int ncu = sh[iup[j]]->cart_size, ncd = idn[j] >= 0 ? sh[idn[j]]->cart_size : 0;
+                        for (int op = 0; op < 4; op++) {
+                            double s = op == 0 ? 1.0 : -1.0;
+                            for (int a = 0; a < ncu; a++)
+                                for (int b = 0; b < nA; b++) tu[op * ncu * nA + a * nA + b] = buf[op * tot + bup[kji] + a * nA + b];
+                            for (int a = 0; a < ncd; a++)
+                                for (int b = 0; b < nA; b++) td[op * ncd * nA + a * nA + b] = buf[op * tot + bdn[kji] + a * nA + b];
+                            (void) s;
+                        }
+                        double *kt = (double *) malloc(sizeof(double) * 12 * nA * nB);
+                        bra_derivative(shells[j]->L, nA, ncd ? td : NULL, (long long) ncd * nA, tu, (long long) ncu * nA, kt);
+                        for (int op = 0; op < 4; op++) {
+                            double s = op == 0 ? 1.0 : -1.0;
+                            for (int k = 0; k < 3; k++)
+                                for (int b = 0; b < nB; b++)
+                                    for (int a = 0; a < nA; a++)
+                                        ket[((op * 3 + k) * nA + a) * nB + b] = s * kt[((op * 3 + k) * nB + b) * nA + a];
+                        }
+                        free(kt);
+                    }
+                    for (int op = 0; op < 4; op++)
+                        for (int k = 0; k < 3; k++) {
+                            const double *br = bra + (op * 3 + k) * nA * nB, *kt = ket + (op * 3 + k) * nA * nB;
+                            double *gA = atom[i] >= 0 ? out + ((long long) (op * 3 * ncenters + 3 * atom[i] + k)) * n2 : NULL;
+                            double *gB = atom[j] >= 0 ? out + ((long long) (op * 3 * ncenters + 3 * atom[j] + k)) * n2 : NULL;
+                            double *gC = out + ((long long) (op * 3 * ncenters + 3 * c + k)) * n2;
+                            for (int a = 0; a < nA; a++)
+                                for (int b = 0; b < nB; b++) {
+                                    long long e = (long long) (off[i] + a) * dim + off[j] + b;
+                                    if (!AC) {
+                                        if (gA) gA[e] += br[a * nB + b];
+                                        gC[e] -= br[a * nB + b];
+                                    }
+                                    if (!BC) {
+                                        if (gB) gB[e] += kt[a * nB + b];
+                                        gC[e] -= kt[a * nB + b];
+                                    }
+                                }
+                        }
+                }
+        free(bra);
+        free(ket);
+        free(tu);
+        free(td);
+    }
+    for (int i = nshells; i < next; i++) libgrpp_delete_shell(sh[i]);
+    free(sh);
+    free(idn);
+    free(iup);
+    free(cidx);
+    free(cs);
+    free(it);
+    free(bup);
+    free(bdn);
+    free(buf);
+    free(atom);
+    free(off);
+    return rc;
+}

@@ -1,0 +1,68 @@
+// Internal interface between the C API layer (api.c) and the HIP engine (engine.hip).
+#pragma once
+
+#include "libgrpp.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* which operator parts a job evaluates */
+enum {
+    ENG_TYPE1 = 1,      /* local U_L (type-1) */
+    ENG_TYPE2 = 2,      /* semilocal AREP channels */
+    ENG_SO = 4,         /* spin-orbit channels, each scaled by so_scale */
+    ENG_OC = 8,         /* outercore projectors */
+    ENG_ALL = 15
+};
+
+typedef struct {
+    const double *origin;               /* [3] */
+    const libgrpp_potential_t *local;   /* may be NULL */
+    int n_arep;
+    libgrpp_potential_t *const *arep;
+    int n_so;
+    libgrpp_potential_t *const *so;
+    const double *so_scale;             /* per SO channel factor (NULL: 2/(2L+1)) */
+    int n_oc;
+    libgrpp_potential_t *const *oc_pot;
+    libgrpp_shell_t *const *oc_shell;   /* centred at origin regardless of their own origin field */
+    int shell_begin, shell_end;         /* shells this centre acts on (end < 0: all) */
+} eng_center_t;
+
+/* one output block: shells (a, b), centres [cb, ce) of the centre-index list, destination
+ * out[op*op_stride + off + i*stride + j]; mirror != 0 also writes the transposed block at moff
+ * (arep symmetric, SO antisymmetric) */
+typedef struct {
+    int a, b, cb, ce;
+    long long off, moff;
+    int stride, mirror;
+} eng_item_t;
+
+typedef struct eng_plan eng_plan_t;
+
+/* flags */
+enum { ENG_F_LEARNED = 1, ENG_F_STATS = 2 };
+
+int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int ncenter, const eng_center_t *centers,
+                    int nidx, const int *center_index, int nitem, const eng_item_t *items, long long out_len,
+                    long long op_stride, int parts, int flags, eng_plan_t **plan);
+int eng_plan_execute(eng_plan_t *plan, void *stream);
+int eng_plan_download(eng_plan_t *plan, double *out);   /* out_len * 4 doubles */
+double *eng_plan_device_output(eng_plan_t *plan);
+void eng_plan_destroy(eng_plan_t *plan);
+int eng_plan_stats(const eng_plan_t *plan, libgrpp_batch_stats_t *stats);
+int eng_plan_features(eng_plan_t *plan, int *n, float *features);
+int eng_plan_set_predictions(eng_plan_t *plan, int n, const float *pred);
+
+/* convenience: create + execute + download + destroy */
+int eng_run(int nshell, libgrpp_shell_t *const *shells, int ncenter, const eng_center_t *centers, int nidx,
+            const int *center_index, int nitem, const eng_item_t *items, long long out_len, long long op_stride,
+            int parts, double *out);
+
+void eng_set_error(const char *fmt, ...);
+int eng_device_ok(void);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,1585 @@
+// MI355X (gfx950) engine for the GRPP integral path.
+//
+// A job is a list of shells, a list of GRPP centres and a list of output blocks.  It runs as a
+// fixed sequence of kernels on one stream:
+//
+//   k_tab_F     F^{C,A}_lambda(r_i): contracted radial functions of every shell A about every
+//               centre C on the level-ordered Log3 grid (radial_type2_integral.c:465-489);
+//   k_tab_wU    w_i U_p(r_i) for every semilocal / spin-orbit channel p (potential.c:70-81);
+//   k_radial    the radial integrals of ALL shell pairs of a centre and channel as one FP64
+//               product Q = G diag(wU) G^T over grid points, rows (shell, lambda, n_A) with
+//               G = r^{n_A} F; the K loop runs level by level (31, 63, ..., 8191 points) and
+//               applies libgrpp's per-entry nested-doubling stop rule at each level boundary
+//               (radial_type2_integral.c:242-318); tiles stop early when every entry stopped;
+//   k_proj      angular projectors P^{C,A,L}[i][m][(lambda,n_A)] = binomial expansion of the
+//               Cartesian component about C times Omega^lambda_{Lm} (type2_integrals.c:117-211,
+//               angular_integrals.c:63-111), and their L-matrix products for spin-orbit;
+//   k_ocv       outercore overlap / type-1 vectors <A|phi_m>, <A|U|phi_m> (outercore_integrals.c:147-270);
+//   k_delta     outercore one-centre radial constants (outercore_integrals.c:397-465);
+//   k_type1     type-1 (local) McMurchie-Davidson blocks (type1_mcmurchie_davidson.c:129-644),
+//               written first into the output blocks;
+//   k_contract  sum over channels of 16 pi^2 sum_m P^A_m Q P^B_m^T (and the SO analogue with
+//               L-matrices) plus the outercore products, added into the output blocks.
+//
+// All arithmetic is FP64.  Every shell-pair block is written by one workgroup, so no atomics
+// are needed on the outputs; blocks of (B,A) are produced from (A,B) by (anti)symmetry.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "device_math.h"
+#include "engine.h"
+#include "tables.h"
+
+using namespace grpp;
+
+// ------------------------------------------------------------------ errors
+static thread_local char g_err[512] = "";
+
+extern "C" void eng_set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+extern "C" const char *libgrpp_last_error(void) { return g_err; }
+
+#define HIPCHK(x)                                                                               \
+    do {                                                                                        \
+        hipError_t e_ = (x);                                                                    \
+        if (e_ != hipSuccess) {                                                                 \
+            eng_set_error("%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return 1;                                                                           \
+        }                                                                                       \
+    } while (0)
+
+// ------------------------------------------------------------------ device-side records
+struct DShell {
+    int L, nprim, poff, pad;
+    double x, y, z;
+};
+
+struct DCenter {
+    double x, y, z;
+    int local;            // potential index of U_L or -1
+    int pb, pe;           // semilocal + SO channels
+    int ob, oe;           // outercore entries (global OC index)
+    int sb, se;           // shells acted on
+    int delta_off;        // offset of the n_oc x n_oc delta block
+    int pad;
+};
+
+struct DPot {
+    int L, J, kind, nterm, toff, slot, M, pad;
+    long long qoff;
+    double scale, thr;
+};
+
+enum { K_LOCAL = 0, K_AREP = 1, K_SO = 2, K_OC = 3 };
+
+struct TabItem {
+    int c, A, frow, lam;
+};
+struct RowDesc {
+    int frow, n;
+};
+struct QJob {
+    int rows, M, pot, maxlev;
+    long long qoff;
+    double thr;
+};
+struct TileItem {
+    int job, ti, tj, pad;
+};
+struct ProjItem {
+    int c, A, L, K;
+    long long poff, ploff;
+};
+struct OcvItem {
+    int c, A, oc, pad;      // oc: global OC index (potential + shell)
+    long long off;
+};
+struct OcEntry {
+    int pot, shell;         // potential index, OC-shell index (in the shell array)
+};
+struct WorkItem {
+    int a, b, cb, ce;
+    long long off, moff;
+    int stride, mirror;
+};
+
+struct DevJob {
+    const DShell *sh;
+    const double *alpha, *craw, *cnorm;
+    const DCenter *ctr;
+    const DPot *pot;
+    const int *tpow;
+    const double *tcoef, *tzeta;
+    const int *cidx;
+    const OcEntry *oce;
+    // lookup tables, slot = (centre, L) pair
+    const int *slot_rowoff;        // [slot * nshell + A]
+    const long long *slot_poff;    // [slot * nshell + A]
+    const long long *slot_ploff;   // [slot * nshell + A] (-1 if the slot has no SO channel)
+    const long long *ocv_tab;      // [oc * nshell + A]
+    const double *delta;
+    const double *Q, *P, *OCV;
+    int nshell;
+    long long op_stride;
+};
+
+// ------------------------------------------------------------------ small helpers
+__device__ __forceinline__ int dev_ncart(int l) { return (l + 1) * (l + 2) / 2; }
+__device__ __forceinline__ int dev_lam_lo(int L, int n) { return (n <= L) ? (L - n) : ((L + n) & 1); }
+__device__ __forceinline__ int dev_lam_cnt(int L, int n) { return (L + n - dev_lam_lo(L, n)) / 2 + 1; }
+__device__ __forceinline__ int dev_kcols(int L, int l)
+{
+    int k = 0;
+    for (int n = 0; n <= l; n++) k += dev_lam_cnt(L, n);
+    return k;
+}
+// Cartesian component c of angular momentum l in generator order (x desc, then y desc)
+__device__ __forceinline__ void dev_cart(int l, int c, int &nx, int &ny, int &nz)
+{
+    int r = l, base = 0;
+    while (c >= base + (l - r + 1)) {
+        base += l - r + 1;
+        r--;
+    }
+    nx = r;
+    ny = l - r - (c - base);
+    nz = l - nx - ny;
+}
+__device__ __forceinline__ int dev_cart_index(int nx, int ny, int nz)
+{
+    int l = nx + ny + nz;
+    return (l - nx) * (l - nx + 1) / 2 + (l - nx - ny);
+}
+__device__ __forceinline__ double dev_binom(int n, int k)
+{
+    if (k < 0 || k > n) return 0.0;
+    double r = 1.0;
+    if (k > n - k) k = n - k;
+    for (int d = 1; d <= k; d++) r = r * (n - k + d) / d;
+    return r;
+}
+__device__ __forceinline__ double gnorm(int L, double a) { return pow(2 * a / M_PI, 0.75) * pow(4 * a, 0.5 * L); }
+
+// ================================================================== tabulation kernels
+constexpr int MAXLAM = LMAX_POT + LMAX_SHELL;   // 20
+
+__global__ __launch_bounds__(256) void k_tab_F(const TabItem *__restrict__ items, DevJob J, DevTables T,
+                                               double *__restrict__ F)
+{
+    const TabItem it = items[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NPTS) return;
+    const DShell s = J.sh[it.A];
+    const DCenter c = J.ctr[it.c];
+    const double dx = c.x - s.x, dy = c.y - s.y, dz = c.z - s.z;
+    const double CA = sqrt(dx * dx + dy * dy + dz * dz);
+    const double r = T.r[i];
+    double acc[MAXLAM + 1];
+#pragma unroll
+    for (int l = 0; l <= MAXLAM; l++) acc[l] = 0.0;
+    const double d2 = (r - CA) * (r - CA);
+    for (int k = 0; k < s.nprim; k++) {
+        const double cf = J.cnorm[s.poff + k];
+        if (cf == 0.0) continue;
+        const double a = J.alpha[s.poff + k];
+        const double e = cf * exp(-a * d2);
+        const double x = 2.0 * a * CA * r;
+        if (x > 1e-7 && x <= 16.0) {
+            const int i0 = (int) (rint(x * 100) + 1e-5);
+            const double ddx = x - i0 * 0.01, dx2 = ddx * ddx, dx3 = dx2 * ddx, dx4 = dx3 * ddx;
+            const double *row = T.bes + (size_t) i0 * BES_NL * 5;
+#pragma unroll
+            for (int l = 0; l <= MAXLAM; l++) {
+                if (l <= it.lam) {
+                    const double *d = row + l * 5;
+                    const double K = d[0] + d[1] * ddx + d[2] * dx2 * 0.5 + d[3] * dx3 * (1.0 / 6.0) +
+                                     d[4] * dx4 * (1.0 / 24.0);
+                    acc[l] += e * K;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int l = 0; l <= MAXLAM; l++)
+                if (l <= it.lam) acc[l] += e * bessel_k(T.bes, T.rfun, l, x);
+        }
+    }
+#pragma unroll
+    for (int l = 0; l <= MAXLAM; l++)
+        if (l <= it.lam) F[(size_t) (it.frow + l) * NPTS + i] = acc[l];
+}
+
+__global__ __launch_bounds__(256) void k_tab_wU(const int *__restrict__ pots, DevJob J, DevTables T,
+                                                double *__restrict__ wU)
+{
+    const int p = pots[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NPTS) return;
+    const DPot P = J.pot[p];
+    const double r = T.r[i], r2 = r * r;
+    double v = 0.0;
+    for (int k = 0; k < P.nterm; k++)
+        v += J.tcoef[P.toff + k] * pow(r, J.tpow[P.toff + k] - 2) * exp(-J.tzeta[P.toff + k] * r2);
+    wU[(size_t) blockIdx.y * NPTS + i] = T.w[i] * v;
+}
+
+// ================================================================== radial quadrature as a tiled product
+constexpr int RT = 64;   // tile edge (entries)
+constexpr int RK = 32;   // grid points per K step
+
+__global__ __launch_bounds__(256) void k_radial(const TileItem *__restrict__ tiles, const QJob *__restrict__ jobs,
+                                                const RowDesc *__restrict__ rows, const double *__restrict__ F,
+                                                const double *__restrict__ rpow, const double *__restrict__ wU,
+                                                double *__restrict__ Q, int min_level, double tol,
+                                                unsigned long long *__restrict__ counters)
+{
+    __shared__ double As[RK][RT + 2];
+    __shared__ double Bs[RK][RT + 2];
+    __shared__ int sfa[RT], sna[RT], sfb[RT], snb[RT];
+    const TileItem t = tiles[blockIdx.x];
+    const QJob jb = jobs[t.job];
+    const int tid = threadIdx.x;
+    const RowDesc *R = rows + jb.rows;
+    if (tid < RT) {
+        int r = t.ti * RT + tid;
+        sfa[tid] = (r < jb.M) ? R[r].frow : -1;
+        sna[tid] = (r < jb.M) ? R[r].n : 0;
+    } else if (tid < 2 * RT) {
+        int r = t.tj * RT + tid - RT;
+        sfb[tid - RT] = (r < jb.M) ? R[r].frow : -1;
+        snb[tid - RT] = (r < jb.M) ? R[r].n : 0;
+    }
+    __syncthreads();
+    const int tx = tid & 15, ty = tid >> 4;
+    const double *wp = wU + (size_t) jb.pot * NPTS;
+
+    double acc[4][4], S[4][4], res[4][4];
+    unsigned done = 0;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            S[a][b] = 0.0;
+            res[a][b] = 0.0;
+            const int row = t.ti * RT + ty * 4 + a, col = t.tj * RT + tx * 4 + b;
+            if (row >= jb.M || col >= jb.M) done |= 1u << (a * 4 + b);
+        }
+
+    int lev_done = 0;
+    for (int lev = 0; lev <= jb.maxlev; lev++) {
+        const int kb = (lev == 0) ? 0 : (16 << lev);
+        const int ke = (lev == 0) ? 32 : (32 << lev);
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
+        for (int k0 = kb; k0 < ke; k0 += RK) {
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int e = tid + 256 * q, r = e >> 5, kk = e & 31, k = k0 + kk;
+                const int fa = sfa[r], fb = sfb[r];
+                As[kk][r] = (fa >= 0) ? F[(size_t) fa * NPTS + k] * rpow[sna[r] * NPTS + k] : 0.0;
+                Bs[kk][r] = (fb >= 0) ? F[(size_t) fb * NPTS + k] * rpow[snb[r] * NPTS + k] * wp[k] : 0.0;
+            }
+            __syncthreads();
+#pragma unroll 8
+            for (int kk = 0; kk < RK; kk++) {
+                double av[4], bv[4];
+#pragma unroll
+                for (int a = 0; a < 4; a++) av[a] = As[kk][ty * 4 + a];
+#pragma unroll
+                for (int b = 0; b < 4; b++) bv[b] = Bs[kk][tx * 4 + b];
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+#pragma unroll
+                    for (int b = 0; b < 4; b++) acc[a][b] = fma(av[a], bv[b], acc[a][b]);
+            }
+            __syncthreads();
+        }
+        // libgrpp's stop rule per entry: S_k = S_{k-1}/2 + (new points), stop once |S_k - S_{k-1}| <= tol
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const unsigned bit = 1u << (a * 4 + b);
+                if (lev == 0) {
+                    S[a][b] = acc[a][b];
+                } else {
+                    const double prev = S[a][b];
+                    S[a][b] = 0.5 * S[a][b] + acc[a][b];
+                    if (!(done & bit) && lev >= min_level && fabs(S[a][b] - prev) <= tol) {
+                        done |= bit;
+                        res[a][b] = S[a][b];
+                    }
+                }
+            }
+        lev_done = lev;
+        if (!__syncthreads_or(done != 0xFFFFu)) break;
+    }
+    if (tid == 0 && counters) {
+        atomicAdd(&counters[0], (unsigned long long) (32ull << lev_done));   // grid points integrated
+        atomicAdd(&counters[1], 1ull);                                       // tiles
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const unsigned bit = 1u << (a * 4 + b);
+            const int row = t.ti * RT + ty * 4 + a, col = t.tj * RT + tx * 4 + b;
+            if (row >= jb.M || col >= jb.M) continue;
+            double v = (done & bit) ? res[a][b] : S[a][b];
+            if (fabs(v) < jb.thr) v = 0.0;   // |Q| skips of type2_integrals.c:185 / spin_orbit_integrals.c:205
+            if (t.ti < t.tj || row <= col) {
+                Q[jb.qoff + (long long) row * jb.M + col] = v;
+                Q[jb.qoff + (long long) col * jb.M + row] = v;
+            }
+        }
+}
+
+// ================================================================== angular projectors
+constexpr int PROJ_LMAX = LMAX_POT + LMAX_SHELL;  // lambda <= L + l
+
+__global__ __launch_bounds__(256) void k_proj(const ProjItem *__restrict__ items, DevJob J, DevTables T,
+                                              double *__restrict__ P)
+{
+    __shared__ double Sk[(PROJ_LMAX + 1) * (2 * PROJ_LMAX + 1)];
+    __shared__ double Om[6144];
+    const ProjItem it = items[blockIdx.x];
+    const DShell s = J.sh[it.A];
+    const DCenter c = J.ctr[it.c];
+    const int L = it.L, la = s.L, nm = 2 * L + 1, nc = dev_ncart(la), K = it.K;
+    const int tid = threadIdx.x;
+    const double CAx = c.x - s.x, CAy = c.y - s.y, CAz = c.z - s.z;
+    // k_A = -2 alpha_0 CA; only its direction enters (type2_integrals.c:66-81)
+    const double a0 = J.alpha[s.poff];
+    const double kx = -2.0 * (a0 * CAx), ky = -2.0 * (a0 * CAy), kz = -2.0 * (a0 * CAz);
+    const double len = sqrt(kx * kx + ky * ky + kz * kz);
+    double ux = 0, uy = 0, uz = 0;
+    if (len > 1e-14) {
+        const double inv = 1.0 / len;
+        ux = kx * inv;
+        uy = ky * inv;
+        uz = kz * inv;
+    }
+    const int lmx = L + la;
+    // real harmonics S_{lambda mu}(k) with the |S| < 1e-14 skip of angular_integrals.c:77
+    for (int e = tid; e < (lmx + 1) * (2 * PROJ_LMAX + 1); e += blockDim.x) {
+        const int lam = e / (2 * PROJ_LMAX + 1), mu = e % (2 * PROJ_LMAX + 1);
+        double v = 0.0;
+        if (mu <= 2 * lam) {
+            const int ncl = (lam + 1) * (lam + 2) / 2;
+            const double *y = T.rsh + T.rsh_off[lam] + mu * ncl;
+            const int *cb = T.comb + T.comb_off[lam];
+            for (int q = 0; q < ncl; q++) {
+                double p = 1.0;
+                for (int z = 0; z < cb[3 * q]; z++) p *= ux;
+                double py = 1.0;
+                for (int z = 0; z < cb[3 * q + 1]; z++) py *= uy;
+                double pz = 1.0;
+                for (int z = 0; z < cb[3 * q + 2]; z++) pz *= uz;
+                v += y[q] * (p * py * pz);
+            }
+            if (fabs(v) < 1e-14) v = 0.0;
+        }
+        Sk[e] = v;
+    }
+    __syncthreads();
+    int colbase = 0;
+    for (int n = 0; n <= la; n++) {
+        const int cnt = dev_lam_cnt(L, n), lo = dev_lam_lo(L, n), nab = (n + 1) * (n + 2) / 2;
+        const int wo = T.wdir[L * (T.w_nmax + 1) + n], ws = T.wstride[L * (T.w_nmax + 1) + n];
+        // Omega[j][m][abc] = sum_mu S_{lam_j mu} W[j][m][mu][abc]
+        const int nom = cnt * nm * nab;
+        for (int e = tid; e < nom; e += blockDim.x) {
+            const int q = e % nab, jm = e / nab, m = jm % nm, j = jm / nm;
+            const int lam = lo + 2 * j;
+            const double *w = T.W + wo + ((size_t) (j * nm + m) * ws) * nab + q;
+            const double *sk = Sk + lam * (2 * PROJ_LMAX + 1);
+            double o = 0.0;
+            for (int mu = 0; mu <= 2 * lam; mu++) o += w[(size_t) mu * nab] * sk[mu];
+            Om[e] = o;
+        }
+        __syncthreads();
+        // P[i][m][colbase + j] = sum_{abc <= cart_i, a+b+c = n} binomials * CA powers * Omega
+        const int np = nc * nm * cnt;
+        for (int e = tid; e < np; e += blockDim.x) {
+            const int j = e % cnt, im = e / cnt, m = im % nm, i = im / nm;
+            int cx, cy, cz;
+            dev_cart(la, i, cx, cy, cz);
+            double v = 0.0;
+            for (int a = 0; a <= cx && a <= n; a++) {
+                const double fa = dev_binom(cx, a) * pow(CAx, cx - a);
+                for (int b = 0; b <= cy && a + b <= n; b++) {
+                    const int cc = n - a - b;
+                    if (cc > cz) continue;
+                    const double f = fa * dev_binom(cy, b) * pow(CAy, cy - b) * dev_binom(cz, cc) * pow(CAz, cz - cc);
+                    const int q = (n - a) * (n - a + 1) / 2 + (n - a - b);
+                    v += f * Om[(j * nm + m) * nab + q];
+                }
+            }
+            P[it.poff + ((long long) i * nm + m) * K + colbase + j] = v;
+        }
+        colbase += cnt;
+        __syncthreads();
+    }
+    if (it.ploff < 0) return;
+    __threadfence();
+    __syncthreads();
+    // ket-side spin-orbit projectors PL_eta[i][m1][k] = sum_m2 L_eta[m1][m2] P[i][m2][k]
+    const double *Lm = T.lmat + T.lmat_off[L];
+    const int tot = 3 * nc * nm * K;
+    for (int e = tid; e < tot; e += blockDim.x) {
+        const int k = e % K, r1 = e / K, m1 = r1 % nm, r2 = r1 / nm, i = r2 % nc, eta = r2 / nc;
+        const double *Le = Lm + eta * nm * nm + m1 * nm;
+        double v = 0.0;
+        for (int m2 = 0; m2 < nm; m2++) v += Le[m2] * P[it.poff + ((long long) i * nm + m2) * K + k];
+        P[it.ploff + e] = v;
+    }
+}
+
+// ================================================================== McMurchie-Davidson blocks (type-1 and outercore)
+// E^{ij}_t for one Cartesian direction (type1_mcmurchie_davidson.c:591-644); layout [i][j][t], dims (la+1)(lb+1)(N+1)
+__device__ void mmd_E_dir(double *E, int la, int lb, double q, double K, double XQA, double XQB)
+{
+    const int N1 = la + lb + 1;
+#define EI(i, j, t) E[((i) * (lb + 1) + (j)) * N1 + (t)]
+    for (int i = 0; i <= la; i++)
+        for (int j = 0; j <= lb; j++)
+            for (int t = 0; t <= i + j; t++) {
+                if (t == 0) {
+                    if (i == 0 && j == 0) EI(0, 0, 0) = K;
+                    else if (i == 1 && j == 0) EI(1, 0, 0) = XQA * EI(0, 0, 0);
+                    else if (i == 0 && j == 1) EI(0, 1, 0) = XQB * EI(0, 0, 0);
+                    else if (i == 0) EI(i, j, 0) = XQB * EI(i, j - 1, 0) + EI(i, j - 1, 1);
+                    else EI(i, j, 0) = XQA * EI(i - 1, j, 0) + EI(i - 1, j, 1);
+                } else {
+                    double e = 0.0, f = 1.0 / (2.0 * q * t);
+                    if (i > 0) e += f * i * EI(i - 1, j, t - 1);
+                    if (j > 0) e += f * j * EI(i, j - 1, t - 1);
+                    EI(i, j, t) = e;
+                }
+            }
+#undef EI
+}
+
+__device__ __forceinline__ int simplex_idx(int t, int u, int v)
+{
+    const int s = t + u + v;
+    return s * (s + 1) * (s + 2) / 6 + t * (s + 1) - t * (t - 1) / 2 + u;
+}
+
+struct MmdShell {
+    int L, nprim, poff;
+    double x, y, z;
+};
+
+// Accumulates sum_k d_k <A| r_C^{n_k-2} e^{-z_k r_C^2} |B> over the terms [t0, t1) into acc
+// (LDS, element i*ncB + j owned by thread (i*ncB+j) % blockDim).  Coefficients are raw; the
+// primitive factors (2a/pi)^{3/4}(4a)^{L/2} are applied as in type1_mcmurchie_davidson.c.
+__device__ void mmd_accumulate(const MmdShell A, const MmdShell B, double Cx, double Cy, double Cz,
+                               const int *tpow, const double *tcoef, const double *tzeta, int t0, int t1,
+                               const double *alpha, const double *craw, double *acc, double *scratch,
+                               int scratch_len, const double *boys)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int la = A.L, lb = B.L, N = la + lb, N1 = N + 1;
+    const int ncA = dev_ncart(la), ncB = dev_ncart(lb), nel = ncA * ncB;
+    const int esz = 3 * (la + 1) * (lb + 1) * N1;   // E for three directions
+    const int ssz = (N + 1) * (N + 2) * (N + 3) / 6;  // simplex layer
+    __shared__ double w_s[64], q_s[64];
+    __shared__ int n_s;
+    for (int tk = t0; tk < t1; tk++) {
+        const int npow = tpow[tk];
+        const double dk = tcoef[tk], zk = tzeta[tk];
+        const int per = (npow == 2) ? esz : esz + 2 * ssz + 64;
+        int bmax = scratch_len / per;
+        if (bmax > 64) bmax = 64;
+        if (npow != 2) bmax = 1;
+        const int npair = A.nprim * B.nprim;
+        for (int p0 = 0; p0 < npair; p0 += bmax) {
+            const int nb = min(bmax, npair - p0);
+            // phase 1: E tables (one thread per (pair, direction))
+            for (int e = tid; e < nb * 3; e += nt) {
+                const int pb = e / 3, dir = e % 3, pp = p0 + pb;
+                const int ia = pp / B.nprim, ib = pp % B.nprim;
+                const double a = alpha[A.poff + ia], b = alpha[B.poff + ib], c = zk;
+                const double q = a + b + c;
+                const double Ad = dir == 0 ? A.x : (dir == 1 ? A.y : A.z);
+                const double Bd = dir == 0 ? B.x : (dir == 1 ? B.y : B.z);
+                const double Cd = dir == 0 ? Cx : (dir == 1 ? Cy : Cz);
+                const double Qd = (a * Ad + b * Bd + c * Cd) / q;
+                const double XAB = Ad - Bd, XAC = Ad - Cd, XBC = Bd - Cd;
+                const double K = exp(-(a * b / q) * XAB * XAB) * exp(-(a * c / q) * XAC * XAC) * exp(-(b * c / q) * XBC * XBC);
+                mmd_E_dir(scratch + pb * per + dir * (la + 1) * (lb + 1) * N1, la, lb, q, K, Qd - Ad, Qd - Bd);
+                if (dir == 0) {
+                    const double ca = craw[A.poff + ia], cb = craw[B.poff + ib];
+                    const double skip = (fabs(ca) < 1e-14 || fabs(cb) < 1e-14) ? 0.0 : 1.0;
+                    const double NA = gnorm(la, a), NB = gnorm(lb, b);
+                    double w;
+                    if (npow == 2) w = NA * NB * pow(M_PI / q, 1.5);
+                    else if (npow == 1) w = NA * NB * 2.0 * M_PI / q;
+                    else w = NA * NB * 2.0 * pow(M_PI, 1.5) / sqrt(q);
+                    w_s[pb] = skip * dk * ca * cb * w;
+                    q_s[pb] = q;
+                }
+            }
+            __syncthreads();
+            if (npow != 2) {
+                // R^{(n)}_{tuv} (n = 1) or G^{(n)}_{tuv} (n = 0), layered from n = N down to 0
+                // (type1_mcmurchie_davidson.c:479-585); layer n uses layer n+1 only
+                double *Lcur = scratch + esz, *Lnxt = scratch + esz + ssz, *aux = scratch + esz + 2 * ssz;
+                const int ia = p0 / B.nprim, ib = p0 % B.nprim;
+                const double a = alpha[A.poff + ia], b = alpha[B.poff + ib], q = a + b + zk;
+                const double Qx = (a * A.x + b * B.x + zk * Cx) / q, Qy = (a * A.y + b * B.y + zk * Cy) / q,
+                             Qz = (a * A.z + b * B.z + zk * Cz) / q;
+                const double X = Qx - Cx, Y = Qy - Cy, Z = Qz - Cz;
+                if (tid == 0) {
+                    const double arg = q * (X * X + Y * Y + Z * Z);
+                    if (npow == 1)
+                        for (int n = 0; n <= N; n++) aux[n] = boys_f(boys, n, arg);
+                    else
+                        gfun_values(arg, N, aux);
+                }
+                __syncthreads();
+                for (int n = N; n >= 0; n--) {
+                    const int lim = N - n;
+                    const double g2 = (npow == 0) ? 2.0 * q : 0.0;
+                    // level by level in t+u+v: the G recursion reads lower (t,u,v) of the same layer
+                    for (int s = 0; s <= lim; s++) {
+                        const int cnt_s = (s + 1) * (s + 2) / 2;
+                        for (int e = tid; e < cnt_s; e += nt) {
+                            int rem = e, t = 0;
+                            while (rem >= s + 1 - t) {
+                                rem -= s + 1 - t;
+                                t++;
+                            }
+                            const int u = rem, v = s - t - u;
+                            double val = 0.0;
+                            if (s == 0) {
+                                val = pow((npow == 1) ? -2.0 * q : 2.0 * q, n) * aux[n];
+                            } else if (t + u == 0) {
+                                if (v > 1) val += (v - 1) * (Lnxt[simplex_idx(t, u, v - 2)] - g2 * Lcur[simplex_idx(t, u, v - 2)]);
+                                val += Z * (Lnxt[simplex_idx(t, u, v - 1)] - g2 * Lcur[simplex_idx(t, u, v - 1)]);
+                            } else if (t == 0) {
+                                if (u > 1) val += (u - 1) * (Lnxt[simplex_idx(t, u - 2, v)] - g2 * Lcur[simplex_idx(t, u - 2, v)]);
+                                val += Y * (Lnxt[simplex_idx(t, u - 1, v)] - g2 * Lcur[simplex_idx(t, u - 1, v)]);
+                            } else {
+                                if (t > 1) val += (t - 1) * (Lnxt[simplex_idx(t - 2, u, v)] - g2 * Lcur[simplex_idx(t - 2, u, v)]);
+                                val += X * (Lnxt[simplex_idx(t - 1, u, v)] - g2 * Lcur[simplex_idx(t - 1, u, v)]);
+                            }
+                            Lcur[simplex_idx(t, u, v)] = val;
+                        }
+                        __syncthreads();
+                    }
+                    // the layer just computed becomes the "n+1" layer of the next step
+                    double *tmp = Lnxt;
+                    Lnxt = Lcur;
+                    Lcur = tmp;
+                }
+                // after the loop the n = 0 layer is in Lnxt
+                for (int e = tid; e < nel; e += nt) {
+                    const int i = e / ncB, j = e % ncB;
+                    int ax, ay, az, bx, by, bz;
+                    dev_cart(la, i, ax, ay, az);
+                    dev_cart(lb, j, bx, by, bz);
+                    const double *Ex = scratch, *Ey = scratch + (la + 1) * (lb + 1) * N1,
+                                 *Ez = scratch + 2 * (la + 1) * (lb + 1) * N1;
+                    double sacc = 0.0;
+                    for (int t = 0; t <= ax + bx; t++) {
+                        const double ex = Ex[(ax * (lb + 1) + bx) * N1 + t];
+                        for (int u = 0; u <= ay + by; u++) {
+                            const double ey = Ey[(ay * (lb + 1) + by) * N1 + u];
+                            for (int v = 0; v <= az + bz; v++) {
+                                const double ez = Ez[(az * (lb + 1) + bz) * N1 + v];
+                                sacc += ex * ey * ez * Lnxt[simplex_idx(t, u, v)];
+                            }
+                        }
+                    }
+                    acc[e] += w_s[0] * sacc;
+                }
+                __syncthreads();
+            } else {
+                for (int e = tid; e < nel; e += nt) {
+                    const int i = e / ncB, j = e % ncB;
+                    int ax, ay, az, bx, by, bz;
+                    dev_cart(la, i, ax, ay, az);
+                    dev_cart(lb, j, bx, by, bz);
+                    const int ox = (ax * (lb + 1) + bx) * N1, oy = (la + 1) * (lb + 1) * N1 + (ay * (lb + 1) + by) * N1,
+                              oz = 2 * (la + 1) * (lb + 1) * N1 + (az * (lb + 1) + bz) * N1;
+                    double sacc = 0.0;
+                    for (int pb = 0; pb < nb; pb++) {
+                        const double *E = scratch + pb * per;
+                        sacc += w_s[pb] * (E[ox] * E[oy] * E[oz]);
+                    }
+                    acc[e] += sacc;
+                }
+                __syncthreads();
+            }
+        }
+    }
+    (void) q_s;
+    (void) n_s;
+}
+
+constexpr int MMD_SCRATCH = 12288;   // doubles of LDS for E / R tables (96 KiB)
+constexpr int MAX_BLOCK = 66 * 66;
+constexpr int MAX_OCB = 66 * 28;   // basis shell x outercore shell (L <= 6)
+
+__global__ __launch_bounds__(256) void k_type1(const WorkItem *__restrict__ items, DevJob J, DevTables T,
+                                               double *__restrict__ out, int parts)
+{
+    __shared__ double scratch[MMD_SCRATCH];
+    __shared__ double acc[MAX_BLOCK];
+    const WorkItem it = items[blockIdx.x];
+    const DShell sa = J.sh[it.a], sb = J.sh[it.b];
+    const int ncA = dev_ncart(sa.L), ncB = dev_ncart(sb.L), nel = ncA * ncB;
+    for (int e = threadIdx.x; e < nel; e += blockDim.x) acc[e] = 0.0;
+    __syncthreads();
+    if (parts & 1) {
+        const MmdShell A = {sa.L, sa.nprim, sa.poff, sa.x, sa.y, sa.z};
+        const MmdShell B = {sb.L, sb.nprim, sb.poff, sb.x, sb.y, sb.z};
+        for (int ci = it.cb; ci < it.ce; ci++) {
+            const DCenter c = J.ctr[J.cidx[ci]];
+            if (c.local < 0) continue;
+            const DPot p = J.pot[c.local];
+            mmd_accumulate(A, B, c.x, c.y, c.z, J.tpow, J.tcoef, J.tzeta, p.toff, p.toff + p.nterm, J.alpha, J.craw,
+                           acc, scratch, MMD_SCRATCH, T.boys);
+        }
+    }
+    for (int e = threadIdx.x; e < nel; e += blockDim.x) {
+        const int i = e / ncB, j = e % ncB;
+        out[it.off + (long long) i * it.stride + j] = acc[e];
+        if (it.mirror) out[it.moff + (long long) j * it.stride + i] = acc[e];
+        for (int op = 1; op < 4; op++) {
+            out[op * J.op_stride + it.off + (long long) i * it.stride + j] = 0.0;
+            if (it.mirror) out[op * J.op_stride + it.moff + (long long) j * it.stride + i] = 0.0;
+        }
+    }
+}
+
+// outercore vectors: [S (ncA x (2L+1)), U (ncA x (2L+1))] per (centre, shell, OC entry)
+__global__ __launch_bounds__(256) void k_ocv(const OcvItem *__restrict__ items, DevJob J, DevTables T,
+                                             double *__restrict__ OCV)
+{
+    __shared__ double scratch[MMD_SCRATCH];
+    __shared__ double bs[MAX_OCB], bu[MAX_OCB];
+    __shared__ int one_pow;
+    __shared__ double one_coef, zero_zeta;
+    const OcvItem it = items[blockIdx.x];
+    const DCenter c = J.ctr[it.c];
+    const OcEntry oe = J.oce[it.oc];
+    const DShell sa = J.sh[it.A], so = J.sh[oe.shell];
+    const DPot p = J.pot[oe.pot];
+    const int ncA = dev_ncart(sa.L), ncO = dev_ncart(so.L), nel = ncA * ncO, L = so.L, nm = 2 * L + 1;
+    for (int e = threadIdx.x; e < nel; e += blockDim.x) {
+        bs[e] = 0.0;
+        bu[e] = 0.0;
+    }
+    if (threadIdx.x == 0) {
+        one_pow = 2;
+        one_coef = 1.0;
+        zero_zeta = 0.0;
+    }
+    __syncthreads();
+    const MmdShell A = {sa.L, sa.nprim, sa.poff, sa.x, sa.y, sa.z};
+    const MmdShell O = {so.L, so.nprim, so.poff, c.x, c.y, c.z};
+    // overlap <A|phi>: the n = 2 kernel with a zero exponent (overlap.c:39-138 gives the same integral)
+    mmd_accumulate(A, O, c.x, c.y, c.z, &one_pow, &one_coef, &zero_zeta, 0, 1, J.alpha, J.craw, bs, scratch,
+                   MMD_SCRATCH, T.boys);
+    mmd_accumulate(A, O, c.x, c.y, c.z, J.tpow, J.tcoef, J.tzeta, p.toff, p.toff + p.nterm, J.alpha, J.craw, bu,
+                   scratch, MMD_SCRATCH, T.boys);
+    __syncthreads();
+    const double *slm = T.slm + T.slm_off[L];
+    for (int e = threadIdx.x; e < ncA * nm; e += blockDim.x) {
+        const int i = e / nm, m = e % nm;
+        double s = 0.0, u = 0.0;
+        for (int q = 0; q < ncO; q++) {
+            s += slm[m * ncO + q] * bs[i * ncO + q];
+            u += slm[m * ncO + q] * bu[i * ncO + q];
+        }
+        OCV[it.off + e] = s;
+        OCV[it.off + ncA * nm + e] = u;
+    }
+}
+
+// one-centre radial constants delta(phi_i, phi_j) = (U1 + U2)/2 (outercore_integrals.c:397-500)
+__device__ double rgto_norm(int L, double a)
+{
+    double df = 1.0;
+    for (int k = 2 * L + 1; k > 1; k -= 2) df *= k;
+    return sqrt(pow(2.0, L + 2) * pow(2.0, L + 1.5) / (df * sqrt(M_PI))) * pow(a, 0.75 + L / 2.0);
+}
+
+__device__ double onecenter_contracted(const DShell &b, const DShell &k, const DPot &p, const DevJob &J)
+{
+    double sum = 0.0;
+    const int L = p.L;
+    for (int i = 0; i < b.nprim; i++) {
+        const double ai = J.alpha[b.poff + i], Ni = rgto_norm(L, ai);
+        for (int j = 0; j < k.nprim; j++) {
+            const double aj = J.alpha[k.poff + j], Nj = rgto_norm(L, aj);
+            const double f = Ni * Nj * J.craw[b.poff + i] * J.craw[k.poff + j];
+            for (int t = 0; t < p.nterm; t++) {
+                const int n = J.tpow[p.toff + t];
+                const double a = ai + aj + J.tzeta[p.toff + t];
+                double u;
+                if (n % 2 == 0) {
+                    const int kk = L + n / 2;
+                    double df = 1.0;
+                    for (int z = 2 * kk - 1; z > 1; z -= 2) df *= z;
+                    u = df / (pow(2.0, kk + 1) * pow(a, kk)) * sqrt(M_PI / a);
+                } else {
+                    const int kk = L + (n - 1) / 2;
+                    double fk = 1.0;
+                    for (int z = 2; z <= kk; z++) fk *= z;
+                    u = fk / (2.0 * pow(a, kk + 1));
+                }
+                sum += f * J.tcoef[p.toff + t] * u;
+            }
+        }
+    }
+    return sum;
+}
+
+__global__ void k_delta(const int *__restrict__ centers, int n, DevJob J, double *__restrict__ delta)
+{
+    const int ci = blockIdx.x;
+    const DCenter c = J.ctr[centers[ci]];
+    const int no = c.oe - c.ob;
+    for (int e = threadIdx.x; e < no * no; e += blockDim.x) {
+        const int i = e / no, j = e % no;
+        const OcEntry oi = J.oce[c.ob + i], oj = J.oce[c.ob + j];
+        const DPot p1 = J.pot[oi.pot], p2 = J.pot[oj.pot];
+        double d = 0.0;
+        if (p1.L == p2.L && p1.J == p2.J) {
+            const DShell s1 = J.sh[oi.shell], s2 = J.sh[oj.shell];
+            d = 0.5 * (onecenter_contracted(s1, s2, p1, J) + onecenter_contracted(s1, s2, p2, J));
+        }
+        delta[c.delta_off + e] = d;
+    }
+    (void) n;
+}
+
+// ================================================================== contraction
+constexpr int CT_EPT = 8;   // output elements per thread (<= 2048 per row chunk)
+
+__device__ __forceinline__ double int_gauss(int n, double a)
+{
+    if (n % 2 == 0) {
+        const int k = n / 2;
+        double df = 1.0;
+        for (int z = 2 * k - 1; z > 1; z -= 2) df *= z;
+        return df / (pow(2.0, k + 1) * pow(a, k)) * sqrt(M_PI / a);
+    }
+    const int k = (n - 1) / 2;
+    double fk = 1.0;
+    for (int z = 2; z <= k; z++) fk *= z;
+    return fk / (2.0 * pow(a, k + 1));
+}
+
+__global__ __launch_bounds__(256) void k_contract(const WorkItem *__restrict__ items, DevJob J, DevTables T,
+                                                  double *__restrict__ out, int parts)
+{
+    __shared__ double Qs[66 * 66];
+    __shared__ double Ts[66 * 66];
+    const WorkItem it = items[blockIdx.x];
+    const DShell sa = J.sh[it.a], sb = J.sh[it.b];
+    const int la = sa.L, lb = sb.L, ncA = dev_ncart(la), ncB = dev_ncart(lb);
+    const int tid = threadIdx.x;
+    const double FOURPI2 = 16.0 * M_PI * M_PI;
+    const int rows_per_chunk = max(1, (CT_EPT * 256) / ncB);
+    for (int r0 = 0; r0 < ncA; r0 += rows_per_chunk) {
+        const int r1 = min(ncA, r0 + rows_per_chunk), nel = (r1 - r0) * ncB;
+        double acc[4][CT_EPT];
+#pragma unroll
+        for (int o = 0; o < 4; o++)
+#pragma unroll
+            for (int q = 0; q < CT_EPT; q++) acc[o][q] = 0.0;
+
+        for (int ci = it.cb; ci < it.ce; ci++) {
+            const int cidx = J.cidx[ci];
+            const DCenter c = J.ctr[cidx];
+            const double CAx = c.x - sa.x, CAy = c.y - sa.y, CAz = c.z - sa.z;
+            const double CBx = c.x - sb.x, CBy = c.y - sb.y, CBz = c.z - sb.z;
+            const double CA2 = CAx * CAx + CAy * CAy + CAz * CAz, CB2 = CBx * CBx + CBy * CBy + CBz * CBz;
+            const bool onec = CA2 < 1e-14 && CB2 < 1e-14;
+            // ---- semilocal and spin-orbit channels ----
+            for (int pi = c.pb; pi < c.pe; pi++) {
+                const DPot p = J.pot[pi];
+                const bool is_so = (p.kind == K_SO);
+                if (is_so && !(parts & 4)) continue;
+                if (!is_so && !(parts & 2)) continue;
+                const int L = p.L, nm = 2 * L + 1;
+                const int KA = dev_kcols(L, la), KB = dev_kcols(L, lb);
+                // radial block Q[(lam1,nA)][(lam2,nB)]
+                if (onec) {
+                    // analytic one-centre branch (radial_type2_integral.c:126-155): only lambda1 = lambda2 = 0
+                    for (int e = tid; e < KA * KB; e += blockDim.x) Qs[e] = 0.0;
+                    __syncthreads();
+                    if (tid < (la + 1) * (lb + 1)) {
+                        const int nA = tid / (lb + 1), nB = tid % (lb + 1);
+                        if (dev_lam_lo(L, nA) == 0 && dev_lam_lo(L, nB) == 0) {
+                            int ka = 0, kb = 0;
+                            for (int n = 0; n < nA; n++) ka += dev_lam_cnt(L, n);
+                            for (int n = 0; n < nB; n++) kb += dev_lam_cnt(L, n);
+                            double v = 0.0;
+                            for (int i = 0; i < sa.nprim; i++) {
+                                const double ci_ = J.craw[sa.poff + i] * gnorm(la, J.alpha[sa.poff + i]);
+                                for (int j = 0; j < sb.nprim; j++) {
+                                    const double cj = J.craw[sb.poff + j] * gnorm(lb, J.alpha[sb.poff + j]);
+                                    for (int k = 0; k < p.nterm; k++) {
+                                        const double pp = J.alpha[sa.poff + i] + J.alpha[sb.poff + j] + J.tzeta[p.toff + k];
+                                        v += ci_ * cj * J.tcoef[p.toff + k] * int_gauss(J.tpow[p.toff + k] + nA + nB, pp);
+                                    }
+                                }
+                            }
+                            if (fabs(v) < p.thr) v = 0.0;
+                            Qs[ka * KB + kb] = v;
+                        }
+                    }
+                } else {
+                    const int ra = J.slot_rowoff[(long long) p.slot * J.nshell + it.a];
+                    const int rb = J.slot_rowoff[(long long) p.slot * J.nshell + it.b];
+                    for (int e = tid; e < KA * KB; e += blockDim.x) {
+                        const int k1 = e / KB, k2 = e % KB;
+                        Qs[e] = J.Q[p.qoff + (long long) (ra + k1) * p.M + rb + k2];
+                    }
+                }
+                __syncthreads();
+                const double *PA = J.P + J.slot_poff[(long long) p.slot * J.nshell + it.a];
+                const double *PB = J.P + J.slot_poff[(long long) p.slot * J.nshell + it.b];
+                const double *PLB = is_so ? J.P + J.slot_ploff[(long long) p.slot * J.nshell + it.b] : nullptr;
+                const double fac = FOURPI2 * p.scale;
+                for (int m = 0; m < nm; m++) {
+                    // T[i][k2] = sum_k1 P^A[i][m][k1] Q[k1][k2]
+                    for (int e = tid; e < (r1 - r0) * KB; e += blockDim.x) {
+                        const int i = r0 + e / KB, k2 = e % KB;
+                        const double *pa = PA + ((long long) i * nm + m) * KA;
+                        double s = 0.0;
+                        for (int k1 = 0; k1 < KA; k1++) s += pa[k1] * Qs[k1 * KB + k2];
+                        Ts[e] = s;
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int q = 0; q < CT_EPT; q++) {
+                        const int e = tid + q * 256;
+                        if (e >= nel) break;
+                        const int i = e / ncB, j = e % ncB;
+                        const double *tr = Ts + i * KB;
+                        if (!is_so) {
+                            const double *pb = PB + ((long long) j * nm + m) * KB;
+                            double s = 0.0;
+                            for (int k2 = 0; k2 < KB; k2++) s += tr[k2] * pb[k2];
+                            acc[0][q] += fac * s;
+                        } else {
+#pragma unroll
+                            for (int eta = 0; eta < 3; eta++) {
+                                const double *pl = PLB + (((long long) eta * ncB + j) * nm + m) * KB;
+                                double s = 0.0;
+                                for (int k2 = 0; k2 < KB; k2++) s += tr[k2] * pl[k2];
+                                acc[1 + eta][q] += fac * s;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            // ---- outercore projectors ----
+            if ((parts & 8) && c.oe > c.ob) {
+                const int no = c.oe - c.ob;
+                for (int oi = 0; oi < no; oi++) {
+                    const OcEntry e1 = J.oce[c.ob + oi];
+                    const DPot p1 = J.pot[e1.pot];
+                    const int L = J.sh[e1.shell].L, nm = 2 * L + 1;
+                    const double Jv = p1.J / 2.0;
+                    const double af = (Jv < L) ? (L / (2.0 * L + 1)) : ((L + 1) / (2.0 * L + 1));
+                    const double ef = (Jv < L) ? (-2.0 / (2 * L + 1)) : (+2.0 / (2 * L + 1));
+                    const double *SA = J.OCV + J.ocv_tab[(long long) (c.ob + oi) * J.nshell + it.a];
+                    const double *UA = SA + ncA * nm;
+                    const double *SB = J.OCV + J.ocv_tab[(long long) (c.ob + oi) * J.nshell + it.b];
+                    const double *UB = SB + ncB * nm;
+                    const double *Lm = T.lmat + T.lmat_off[L];
+#pragma unroll
+                    for (int q = 0; q < CT_EPT; q++) {
+                        const int e = tid + q * 256;
+                        if (e >= nel) break;
+                        const int i = r0 + e / ncB, j = e % ncB;
+                        double s = 0.0;
+                        for (int m = 0; m < nm; m++) s += SA[i * nm + m] * UB[j * nm + m] + UA[i * nm + m] * SB[j * nm + m];
+                        acc[0][q] += af * s;
+#pragma unroll
+                        for (int eta = 0; eta < 3; eta++) {
+                            const double *Le = Lm + eta * nm * nm;
+                            double so = 0.0;
+                            for (int m1 = 0; m1 < nm; m1++) {
+                                double lsb = 0.0, lub = 0.0;
+                                for (int m2 = 0; m2 < nm; m2++) {
+                                    lsb += Le[m1 * nm + m2] * SB[j * nm + m2];
+                                    lub += Le[m1 * nm + m2] * UB[j * nm + m2];
+                                }
+                                so += UA[i * nm + m1] * lsb + SA[i * nm + m1] * lub;
+                            }
+                            acc[1 + eta][q] += ef * so;
+                        }
+                        // part 2: -P_nlj [(U + U')/2] P_n'lj for OC entries of equal (L, J)
+                        for (int oj = 0; oj < no; oj++) {
+                            const double d = J.delta[c.delta_off + oi * no + oj];
+                            if (d == 0.0) continue;
+                            const double *SB2 = J.OCV + J.ocv_tab[(long long) (c.ob + oj) * J.nshell + it.b];
+                            double s2 = 0.0;
+                            for (int m = 0; m < nm; m++) s2 += SA[i * nm + m] * SB2[j * nm + m];
+                            acc[0][q] += (-1.0) * d * af * s2;
+#pragma unroll
+                            for (int eta = 0; eta < 3; eta++) {
+                                const double *Le = Lm + eta * nm * nm;
+                                double so = 0.0;
+                                for (int m1 = 0; m1 < nm; m1++) {
+                                    double l2 = 0.0;
+                                    for (int m2 = 0; m2 < nm; m2++) l2 += Le[m1 * nm + m2] * SB2[j * nm + m2];
+                                    so += SA[i * nm + m1] * l2;
+                                }
+                                acc[1 + eta][q] += (-1.0) * d * ef * so;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        // ---- accumulate into the output blocks written by k_type1 ----
+#pragma unroll
+        for (int q = 0; q < CT_EPT; q++) {
+            const int e = tid + q * 256;
+            if (e >= nel) break;
+            const int i = r0 + e / ncB, j = e % ncB;
+#pragma unroll
+            for (int op = 0; op < 4; op++) {
+                out[op * J.op_stride + it.off + (long long) i * it.stride + j] += acc[op][q];
+                if (it.mirror)
+                    out[op * J.op_stride + it.moff + (long long) j * it.stride + i] += (op == 0 ? 1.0 : -1.0) * acc[op][q];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ================================================================== device tables (per process, current device)
+struct DeviceTables {
+    DevTables t{};
+    std::vector<void *> bufs;
+    int Lmax = -1, nmax = -1;
+    bool ok = false;
+};
+
+static std::mutex g_tab_mutex;
+static std::map<int, DeviceTables> g_dev_tables;
+
+template <class T>
+static int upload(const std::vector<T> &v, T **out, std::vector<void *> &keep)
+{
+    size_t n = std::max<size_t>(1, v.size()) * sizeof(T);
+    HIPCHK(hipMalloc((void **) out, n));
+    keep.push_back(*out);
+    if (!v.empty()) HIPCHK(hipMemcpy(*out, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+static int device_tables(int Lmax, int nmax, DevTables *out)
+{
+    std::lock_guard<std::mutex> lk(g_tab_mutex);
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    DeviceTables &D = g_dev_tables[dev];
+    if (!D.ok) {
+        const HostTables &H = host_tables();
+        double *p;
+        int *ip;
+        if (upload(H.r, &p, D.bufs)) return 1;
+        D.t.r = p;
+        if (upload(H.w, &p, D.bufs)) return 1;
+        D.t.w = p;
+        if (upload(H.rpow, &p, D.bufs)) return 1;
+        D.t.rpow = p;
+        if (upload(H.bes, &p, D.bufs)) return 1;
+        D.t.bes = p;
+        if (upload(H.rfun, &p, D.bufs)) return 1;
+        D.t.rfun = p;
+        if (upload(H.boys, &p, D.bufs)) return 1;
+        D.t.boys = p;
+        if (upload(H.rsh, &p, D.bufs)) return 1;
+        D.t.rsh = p;
+        if (upload(H.rsh_off, &ip, D.bufs)) return 1;
+        D.t.rsh_off = ip;
+        if (upload(H.comb, &ip, D.bufs)) return 1;
+        D.t.comb = ip;
+        if (upload(H.comb_off, &ip, D.bufs)) return 1;
+        D.t.comb_off = ip;
+        if (upload(H.lmat, &p, D.bufs)) return 1;
+        D.t.lmat = p;
+        if (upload(H.lmat_off, &ip, D.bufs)) return 1;
+        D.t.lmat_off = ip;
+        if (upload(H.slm, &p, D.bufs)) return 1;
+        D.t.slm = p;
+        if (upload(H.slm_off, &ip, D.bufs)) return 1;
+        D.t.slm_off = ip;
+        D.ok = true;
+    }
+    if (Lmax > D.Lmax || nmax > D.nmax) {
+        const int L2 = std::max(Lmax, D.Lmax), n2 = std::max(nmax, D.nmax);
+        std::vector<double> W;
+        std::vector<WBlock> dir;
+        angular_w_tables(L2, n2, W, dir);
+        std::vector<int> wd((L2 + 1) * (n2 + 1), -1), ws((L2 + 1) * (n2 + 1), 0);
+        for (auto &b : dir) {
+            wd[b.L * (n2 + 1) + b.n] = b.off;
+            ws[b.L * (n2 + 1) + b.n] = b.mu_stride;
+        }
+        double *p;
+        int *ip;
+        if (upload(W, &p, D.bufs)) return 1;
+        D.t.W = p;
+        if (upload(wd, &ip, D.bufs)) return 1;
+        D.t.wdir = ip;
+        if (upload(ws, &ip, D.bufs)) return 1;
+        D.t.wstride = ip;
+        D.t.w_nmax = n2;
+        D.Lmax = L2;
+        D.nmax = n2;
+    }
+    *out = D.t;
+    return 0;
+}
+
+extern "C" int eng_device_ok(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        eng_set_error("no HIP device available: the GRPP integrals run only on the GPU");
+        return 0;
+    }
+    return 1;
+}
+
+// ================================================================== plan
+struct eng_plan {
+    int nshell = 0, ncenter = 0, parts = 0, flags = 0;
+    long long out_len = 0, op_stride = 0;
+    DevTables T{};
+    DevJob J{};
+    std::vector<void *> bufs;
+    double *d_out = nullptr, *d_F = nullptr, *d_wU = nullptr, *d_Q = nullptr, *d_P = nullptr, *d_OCV = nullptr,
+           *d_delta = nullptr;
+    TabItem *d_tab = nullptr;
+    int n_tab = 0;
+    int *d_wpots = nullptr;
+    int n_wpots = 0;
+    RowDesc *d_rows = nullptr;
+    QJob *d_jobs = nullptr;
+    TileItem *d_tiles = nullptr;
+    int n_tiles = 0;
+    ProjItem *d_proj = nullptr;
+    int n_proj = 0;
+    OcvItem *d_ocvi = nullptr;
+    int n_ocv = 0;
+    int *d_delta_centers = nullptr;
+    int n_delta_centers = 0;
+    WorkItem *d_items = nullptr;
+    int n_items = 0;
+    unsigned long long *d_counters = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // statistics
+    long long n_tasks = 0, n_integrals = 0, n_entries = 0;
+    double flops_fixed = 0.0;    // contraction + type-1 + projector work (grid independent)
+    double flops_per_point = 0.0; // radial FLOPs per grid point summed over tiles (x points below)
+    unsigned long long last_points = 0, last_tiles = 0;
+    double last_seconds = 0.0, last_radial_seconds = 0.0;
+    int min_level = 0;
+    double tol = 1e-16;
+};
+
+template <class T>
+static int dalloc(eng_plan *P, T **p, size_t n)
+{
+    HIPCHK(hipMalloc((void **) p, std::max<size_t>(1, n) * sizeof(T)));
+    P->bufs.push_back(*p);
+    return 0;
+}
+
+template <class T>
+static int dput(eng_plan *P, T **p, const std::vector<T> &v)
+{
+    if (dalloc(P, p, v.size())) return 1;
+    if (!v.empty()) HIPCHK(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+static int kcols_host(int L, int l)
+{
+    int k = 0;
+    for (int n = 0; n <= l; n++) k += lam_cnt(L, n);
+    return k;
+}
+
+extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int ncenter, const eng_center_t *centers,
+                               int nidx, const int *center_index, int nitem, const eng_item_t *items,
+                               long long out_len, long long op_stride, int parts, int flags, eng_plan_t **plan_out)
+{
+    *plan_out = nullptr;
+    if (!eng_device_ok()) return 1;
+    eng_plan *P = new eng_plan();
+    P->nshell = nshell;
+    P->ncenter = ncenter;
+    P->parts = parts;
+    P->flags = flags;
+    P->out_len = out_len;
+    P->op_stride = op_stride;
+
+    // ---------------- pack shells (basis shells first, then outercore shells) ----------------
+    std::vector<DShell> sh;
+    std::vector<double> alpha, craw, cnorm;
+    int lmax_shell = 0;
+    auto add_shell = [&](const libgrpp_shell_t *s, const double *origin) {
+        DShell d{};
+        d.L = s->L;
+        d.nprim = s->num_primitives;
+        d.poff = (int) alpha.size();
+        d.x = origin[0];
+        d.y = origin[1];
+        d.z = origin[2];
+        for (int k = 0; k < s->num_primitives; k++) {
+            alpha.push_back(s->alpha[k]);
+            craw.push_back(s->coeffs[k]);
+            // libgrpp_shell_shrink + libgrpp_shell_mult_normcoef (shell.c:67-92)
+            cnorm.push_back(std::fabs(s->coeffs[k]) > 1e-14 ? s->coeffs[k] * gaussian_norm(s->L, s->alpha[k]) : 0.0);
+        }
+        sh.push_back(d);
+        return (int) sh.size() - 1;
+    };
+    for (int i = 0; i < nshell; i++) {
+        const libgrpp_shell_t *s = shells[i];
+        if (s->L < 0 || s->L > LMAX_SHELL || s->num_primitives <= 0) {
+            eng_set_error("shell %d: L=%d / %d primitives outside the supported range (L <= %d)", i, s->L,
+                          s->num_primitives, LMAX_SHELL);
+            delete P;
+            return 2;
+        }
+        add_shell(s, s->origin);
+        lmax_shell = std::max(lmax_shell, s->L);
+    }
+
+    // ---------------- potentials ----------------
+    std::vector<DPot> pots;
+    std::vector<int> tpow;
+    std::vector<double> tcoef, tzeta;
+    auto add_pot = [&](const libgrpp_potential_t *p, int kind, double scale) {
+        DPot d{};
+        d.L = p->L;
+        d.J = p->J;
+        d.kind = kind;
+        d.toff = (int) tpow.size();
+        d.slot = -1;
+        d.qoff = -1;
+        d.scale = scale;
+        d.thr = (kind == K_SO) ? 1e-14 : 1e-16;
+        for (int k = 0; k < p->num_primitives; k++) {
+            if (std::fabs(p->coeffs[k]) <= 1e-14) continue;   // libgrpp_shrink_potential (potential.c:88-115)
+            tpow.push_back(p->powers[k]);
+            tcoef.push_back(p->coeffs[k]);
+            tzeta.push_back(p->alpha[k]);
+        }
+        d.nterm = (int) tpow.size() - d.toff;
+        pots.push_back(d);
+        return (int) pots.size() - 1;
+    };
+
+    std::vector<DCenter> ctr(ncenter);
+    std::vector<OcEntry> oce;
+    int Lmax_pot = 0;
+    for (int c = 0; c < ncenter; c++) {
+        const eng_center_t &E = centers[c];
+        DCenter &d = ctr[c];
+        d.x = E.origin[0];
+        d.y = E.origin[1];
+        d.z = E.origin[2];
+        d.sb = E.shell_begin < 0 ? 0 : E.shell_begin;
+        d.se = E.shell_end < 0 ? nshell : E.shell_end;
+        d.local = (E.local && (parts & ENG_TYPE1)) ? add_pot(E.local, K_LOCAL, 1.0) : -1;
+        d.pb = (int) pots.size();
+        if (parts & ENG_TYPE2)
+            for (int k = 0; k < E.n_arep; k++) add_pot(E.arep[k], K_AREP, 1.0);
+        if (parts & ENG_SO)
+            for (int k = 0; k < E.n_so; k++) {
+                const int L = E.so[k]->L;
+                add_pot(E.so[k], K_SO, E.so_scale ? E.so_scale[k] : 2.0 / (2 * L + 1));
+            }
+        d.pe = (int) pots.size();
+        for (int k = d.pb; k < d.pe; k++) {
+            if (pots[k].L < 0 || pots[k].L > LMAX_POT || pots[k].L + lmax_shell > ANG_SUM_MAX) {
+                eng_set_error("channel L=%d with basis L=%d exceeds the supported L + l <= %d", pots[k].L, lmax_shell,
+                              ANG_SUM_MAX);
+                delete P;
+                return 2;
+            }
+            Lmax_pot = std::max(Lmax_pot, pots[k].L);
+        }
+        d.ob = (int) oce.size();
+        if (parts & ENG_OC)
+            for (int k = 0; k < E.n_oc; k++) {
+                OcEntry oe;
+                oe.pot = add_pot(E.oc_pot[k], K_OC, 1.0);
+                oe.shell = add_shell(E.oc_shell[k], E.origin);
+                if (E.oc_shell[k]->L > 6) {
+                    eng_set_error("outercore shell L=%d too large", E.oc_shell[k]->L);
+                    delete P;
+                    return 2;
+                }
+                oce.push_back(oe);
+            }
+        d.oe = (int) oce.size();
+    }
+
+    // ---------------- radial machinery: slots (centre, L) ----------------
+    struct Slot {
+        int c, L, has_so;
+    };
+    std::vector<Slot> slots;
+    std::vector<int> tabF_base;   // per (centre c, shell A): first F row, or -1
+    std::vector<TabItem> tab;
+    long long frows = 0;
+    std::vector<int> slot_of_pot(pots.size(), -1);
+    for (int c = 0; c < ncenter; c++) {
+        DCenter &d = ctr[c];
+        std::map<int, int> Ls;
+        for (int k = d.pb; k < d.pe; k++) {
+            int &flag = Ls[pots[k].L];
+            if (pots[k].kind == K_SO) flag = 1;
+        }
+        std::map<int, int> slot_id;
+        for (auto &kv : Ls) {
+            slot_id[kv.first] = (int) slots.size();
+            slots.push_back({c, kv.first, kv.second});
+        }
+        for (int k = d.pb; k < d.pe; k++) {
+            pots[k].slot = slot_id[pots[k].L];
+        }
+    }
+    const int nslot = (int) slots.size();
+    // F rows per (centre, shell): lambda = 0 .. Lmax_c + l_A
+    std::vector<int> cLmax(ncenter, -1);
+    for (auto &s : slots) cLmax[s.c] = std::max(cLmax[s.c], s.L);
+    tabF_base.assign((size_t) ncenter * nshell, -1);
+    for (int c = 0; c < ncenter; c++) {
+        if (cLmax[c] < 0) continue;
+        for (int A = ctr[c].sb; A < ctr[c].se; A++) {
+            TabItem t;
+            t.c = c;
+            t.A = A;
+            t.frow = (int) frows;
+            t.lam = cLmax[c] + sh[A].L;
+            tabF_base[(size_t) c * nshell + A] = t.frow;
+            frows += t.lam + 1;
+            tab.push_back(t);
+        }
+    }
+    // rows of each slot: shells in order, columns (n_A, lambda)
+    std::vector<RowDesc> rows;
+    std::vector<int> slot_rows(nslot), slot_M(nslot);
+    std::vector<int> slot_rowoff((size_t) nslot * nshell, -1);
+    std::vector<long long> slot_poff((size_t) nslot * nshell, -1), slot_ploff((size_t) nslot * nshell, -1);
+    std::vector<ProjItem> proj;
+    long long plen = 0;
+    for (int s = 0; s < nslot; s++) {
+        const int c = slots[s].c, L = slots[s].L;
+        slot_rows[s] = (int) rows.size();
+        int M = 0;
+        for (int A = ctr[c].sb; A < ctr[c].se; A++) {
+            slot_rowoff[(size_t) s * nshell + A] = M;
+            const int fb = tabF_base[(size_t) c * nshell + A];
+            for (int n = 0; n <= sh[A].L; n++)
+                for (int j = 0; j < lam_cnt(L, n); j++) {
+                    RowDesc r;
+                    r.frow = fb + lam_lo(L, n) + 2 * j;
+                    r.n = n;
+                    rows.push_back(r);
+                    M++;
+                }
+            ProjItem pi;
+            pi.c = c;
+            pi.A = A;
+            pi.L = L;
+            pi.K = kcols_host(L, sh[A].L);
+            pi.poff = plen;
+            plen += (long long) ncart(sh[A].L) * (2 * L + 1) * pi.K;
+            pi.ploff = -1;
+            if (slots[s].has_so) {
+                pi.ploff = plen;
+                plen += 3LL * ncart(sh[A].L) * (2 * L + 1) * pi.K;
+            }
+            slot_poff[(size_t) s * nshell + A] = pi.poff;
+            slot_ploff[(size_t) s * nshell + A] = pi.ploff;
+            proj.push_back(pi);
+        }
+        slot_M[s] = M;
+    }
+    // radial jobs = semilocal / SO channels; tiles over the upper triangle
+    std::vector<QJob> jobs;
+    std::vector<TileItem> tiles;
+    std::vector<int> wpots;
+    long long qlen = 0;
+    double flops_pp = 0.0;
+    for (int k = 0; k < (int) pots.size(); k++) {
+        DPot &p = pots[k];
+        if (p.slot < 0) continue;
+        QJob q;
+        q.rows = slot_rows[p.slot];
+        q.M = slot_M[p.slot];
+        q.pot = (int) wpots.size();
+        q.maxlev = NLEV - 1;
+        q.qoff = qlen;
+        q.thr = p.thr;
+        p.qoff = qlen;
+        p.M = q.M;
+        qlen += (long long) q.M * q.M;
+        wpots.push_back(k);
+        const int nt = (q.M + RT - 1) / RT;
+        for (int ti = 0; ti < nt; ti++)
+            for (int tj = ti; tj < nt; tj++) tiles.push_back({(int) jobs.size(), ti, tj, 0});
+        jobs.push_back(q);
+        P->n_entries += (long long) q.M * (q.M + 1) / 2;
+        flops_pp += 2.0 * RT * RT * nt * (nt + 1) / 2;
+        (void) flops_pp;
+    }
+    // outercore vectors and deltas
+    std::vector<OcvItem> ocv;
+    std::vector<long long> ocv_tab((size_t) std::max<size_t>(1, oce.size()) * nshell, -1);
+    long long ocvlen = 0;
+    std::vector<int> delta_centers;
+    int delta_len = 0;
+    for (int c = 0; c < ncenter; c++) {
+        DCenter &d = ctr[c];
+        const int no = d.oe - d.ob;
+        d.delta_off = delta_len;
+        if (no > 0) {
+            delta_centers.push_back(c);
+            delta_len += no * no;
+        }
+        for (int o = d.ob; o < d.oe; o++)
+            for (int A = d.sb; A < d.se; A++) {
+                OcvItem it;
+                it.c = c;
+                it.A = A;
+                it.oc = o;
+                it.off = ocvlen;
+                ocv_tab[(size_t) o * nshell + A] = ocvlen;
+                ocvlen += 2LL * ncart(sh[A].L) * (2 * sh[oce[o].shell].L + 1);
+                ocv.push_back(it);
+            }
+    }
+    // work items
+    std::vector<WorkItem> witems(nitem);
+    for (int i = 0; i < nitem; i++) {
+        const eng_item_t &e = items[i];
+        if (ncart(sh[e.a].L) * ncart(sh[e.b].L) > MAX_BLOCK) {
+            eng_set_error("shell pair block too large");
+            delete P;
+            return 2;
+        }
+        witems[i] = {e.a, e.b, e.cb, e.ce, e.off, e.moff, e.stride, e.mirror};
+        const long long ntask = (long long) (e.ce - e.cb) * (e.mirror ? 2 : 1);
+        P->n_tasks += ntask;
+        P->n_integrals += 4LL * ncart(sh[e.a].L) * ncart(sh[e.b].L) * ntask;
+    }
+    std::vector<int> cidx(center_index, center_index + nidx);
+
+    // ---------------- upload ----------------
+    if (device_tables(Lmax_pot, lmax_shell, &P->T)) {
+        delete P;
+        return 1;
+    }
+    DShell *d_sh;
+    double *d_alpha, *d_craw, *d_cnorm, *d_tcoef, *d_tzeta;
+    DCenter *d_ctr;
+    DPot *d_pot;
+    int *d_tpow, *d_cidx, *d_slot_rowoff;
+    OcEntry *d_oce;
+    long long *d_slot_poff, *d_slot_ploff, *d_ocv_tab;
+    int rc = 0;
+    rc |= dput(P, &d_sh, sh);
+    rc |= dput(P, &d_alpha, alpha);
+    rc |= dput(P, &d_craw, craw);
+    rc |= dput(P, &d_cnorm, cnorm);
+    rc |= dput(P, &d_ctr, ctr);
+    rc |= dput(P, &d_pot, pots);
+    rc |= dput(P, &d_tpow, tpow);
+    rc |= dput(P, &d_tcoef, tcoef);
+    rc |= dput(P, &d_tzeta, tzeta);
+    rc |= dput(P, &d_cidx, cidx);
+    rc |= dput(P, &d_oce, oce);
+    rc |= dput(P, &d_slot_rowoff, slot_rowoff);
+    rc |= dput(P, &d_slot_poff, slot_poff);
+    rc |= dput(P, &d_slot_ploff, slot_ploff);
+    rc |= dput(P, &d_ocv_tab, ocv_tab);
+    rc |= dput(P, &P->d_tab, tab);
+    rc |= dput(P, &P->d_wpots, wpots);
+    rc |= dput(P, &P->d_rows, rows);
+    rc |= dput(P, &P->d_jobs, jobs);
+    rc |= dput(P, &P->d_tiles, tiles);
+    rc |= dput(P, &P->d_proj, proj);
+    rc |= dput(P, &P->d_ocvi, ocv);
+    rc |= dput(P, &P->d_delta_centers, delta_centers);
+    rc |= dput(P, &P->d_items, witems);
+    rc |= dalloc(P, &P->d_F, (size_t) frows * NPTS);
+    rc |= dalloc(P, &P->d_wU, (size_t) wpots.size() * NPTS);
+    rc |= dalloc(P, &P->d_Q, (size_t) qlen);
+    rc |= dalloc(P, &P->d_P, (size_t) plen);
+    rc |= dalloc(P, &P->d_OCV, (size_t) ocvlen);
+    rc |= dalloc(P, &P->d_delta, (size_t) delta_len);
+    rc |= dalloc(P, &P->d_out, (size_t) 4 * op_stride);
+    rc |= dalloc(P, &P->d_counters, 4);
+    if (rc) {
+        eng_plan_destroy(P);
+        return 1;
+    }
+    P->n_tab = (int) tab.size();
+    P->n_wpots = (int) wpots.size();
+    P->n_tiles = (int) tiles.size();
+    P->n_proj = (int) proj.size();
+    P->n_ocv = (int) ocv.size();
+    P->n_delta_centers = (int) delta_centers.size();
+    P->n_items = nitem;
+
+    DevJob &J = P->J;
+    J.sh = d_sh;
+    J.alpha = d_alpha;
+    J.craw = d_craw;
+    J.cnorm = d_cnorm;
+    J.ctr = d_ctr;
+    J.pot = d_pot;
+    J.tpow = d_tpow;
+    J.tcoef = d_tcoef;
+    J.tzeta = d_tzeta;
+    J.cidx = d_cidx;
+    J.oce = d_oce;
+    J.slot_rowoff = d_slot_rowoff;
+    J.slot_poff = d_slot_poff;
+    J.slot_ploff = d_slot_ploff;
+    J.ocv_tab = d_ocv_tab;
+    J.delta = P->d_delta;
+    J.Q = P->d_Q;
+    J.P = P->d_P;
+    J.OCV = P->d_OCV;
+    J.nshell = nshell;   // lookup tables are indexed by basis shell; outercore shells follow them in sh
+    J.op_stride = op_stride;
+    for (int i = 0; i < 4; i++) HIPCHK(hipEventCreate(&P->ev[i]));
+    *plan_out = P;
+    return 0;
+}
+
+extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
+{
+    hipStream_t st = (hipStream_t) stream_;
+    HIPCHK(hipMemsetAsync(P->d_counters, 0, 4 * sizeof(unsigned long long), st));
+    HIPCHK(hipEventRecord(P->ev[0], st));
+    if (P->n_tab > 0) {
+        dim3 g(NPTS / 256, P->n_tab);
+        hipLaunchKernelGGL(k_tab_F, g, dim3(256), 0, st, P->d_tab, P->J, P->T, P->d_F);
+    }
+    if (P->n_wpots > 0) {
+        dim3 g(NPTS / 256, P->n_wpots);
+        hipLaunchKernelGGL(k_tab_wU, g, dim3(256), 0, st, P->d_wpots, P->J, P->T, P->d_wU);
+    }
+    HIPCHK(hipEventRecord(P->ev[1], st));
+    if (P->n_tiles > 0)
+        hipLaunchKernelGGL(k_radial, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->d_F,
+                           P->T.rpow, P->d_wU, P->d_Q, P->min_level, P->tol, P->d_counters);
+    HIPCHK(hipEventRecord(P->ev[2], st));
+    if (P->n_proj > 0) hipLaunchKernelGGL(k_proj, dim3(P->n_proj), dim3(256), 0, st, P->d_proj, P->J, P->T, P->d_P);
+    if (P->n_delta_centers > 0)
+        hipLaunchKernelGGL(k_delta, dim3(P->n_delta_centers), dim3(64), 0, st, P->d_delta_centers,
+                           P->n_delta_centers, P->J, P->d_delta);
+    if (P->n_ocv > 0) hipLaunchKernelGGL(k_ocv, dim3(P->n_ocv), dim3(256), 0, st, P->d_ocvi, P->J, P->T, P->d_OCV);
+    if (P->n_items > 0) {
+        hipLaunchKernelGGL(k_type1, dim3(P->n_items), dim3(256), 0, st, P->d_items, P->J, P->T, P->d_out, P->parts);
+        hipLaunchKernelGGL(k_contract, dim3(P->n_items), dim3(256), 0, st, P->d_items, P->J, P->T, P->d_out,
+                           P->parts);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(P->ev[3], st));
+    HIPCHK(hipEventSynchronize(P->ev[3]));
+    float ms = 0, msr = 0;
+    HIPCHK(hipEventElapsedTime(&ms, P->ev[0], P->ev[3]));
+    HIPCHK(hipEventElapsedTime(&msr, P->ev[1], P->ev[2]));
+    P->last_seconds = ms * 1e-3;
+    P->last_radial_seconds = msr * 1e-3;
+    unsigned long long cnt[4];
+    HIPCHK(hipMemcpy(cnt, P->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    P->last_points = cnt[0];
+    P->last_tiles = cnt[1];
+    return 0;
+}
+
+extern "C" int eng_plan_download(eng_plan_t *P, double *out)
+{
+    HIPCHK(hipMemcpy(out, P->d_out, (size_t) 4 * P->op_stride * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" double *eng_plan_device_output(eng_plan_t *P) { return P->d_out; }
+
+extern "C" void eng_plan_destroy(eng_plan_t *P)
+{
+    if (!P) return;
+    for (void *b : P->bufs) (void) hipFree(b);
+    for (int i = 0; i < 4; i++)
+        if (P->ev[i]) (void) hipEventDestroy(P->ev[i]);
+    delete P;
+}
+
+extern "C" int eng_plan_stats(const eng_plan_t *P, libgrpp_batch_stats_t *s)
+{
+    memset(s, 0, sizeof(*s));
+    s->n_tasks = P->n_tasks;
+    s->n_integrals = P->n_integrals;
+    s->n_radial_entries = P->n_entries;
+    s->n_radial_points = (long long) P->last_points;
+    s->n_tiles = (long long) P->last_tiles;
+    // radial product: 2 FLOP per (entry, point) over the 64x64 tiles actually integrated
+    s->fp64_flops = 2.0 * RT * RT * (double) P->last_points;
+    s->device_seconds = P->last_seconds;
+    s->radial_seconds = P->last_radial_seconds;
+    return 0;
+}
+
+extern "C" int eng_plan_features(eng_plan_t *, int *n, float *)
+{
+    *n = 0;
+    return 0;
+}
+
+extern "C" int eng_plan_set_predictions(eng_plan_t *, int, const float *) { return 0; }
+
+extern "C" int eng_run(int nshell, libgrpp_shell_t *const *shells, int ncenter, const eng_center_t *centers, int nidx,
+                       const int *center_index, int nitem, const eng_item_t *items, long long out_len,
+                       long long op_stride, int parts, double *out)
+{
+    eng_plan_t *P = nullptr;
+    int rc = eng_plan_create(nshell, shells, ncenter, centers, nidx, center_index, nitem, items, out_len, op_stride,
+                             parts, 0, &P);
+    if (rc) return rc;
+    rc = eng_plan_execute(P, nullptr);
+    if (!rc) rc = eng_plan_download(P, out);
+    eng_plan_destroy(P);
+    return rc;
+}

@@ -1,0 +1,245 @@
+"""Python host interface of libgrpp_amd.so (ctypes), mirroring libgrpp's C API.
+
+The library is loaded from the package folder; if it is missing or no HIP device is usable,
+calls raise -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+from .abi import LIBGRPP_NAMES, PGRPP, PPOT, PSHELL, ObjectFactory, as_dptr, darr, dptr
+from .io import GRPP, Potential, Shell, System
+
+_LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgrpp_amd.so")
+_lib = None
+
+
+class Stats(C.Structure):
+    _fields_ = [("n_tasks", C.c_longlong), ("n_integrals", C.c_longlong), ("n_radial_entries", C.c_longlong),
+                ("n_radial_points", C.c_longlong), ("n_fallback_tiles", C.c_longlong), ("n_tiles", C.c_longlong),
+                ("fp64_flops", C.c_double), ("device_seconds", C.c_double), ("radial_seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+BATCH_LEARNED_GRID = 1
+BATCH_GRADIENT = 2
+BATCH_NO_SYMMETRY = 4
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            raise RuntimeError(f"{_LIB} is not built: run `python -m libgrpp_amd.build` (no CPU fallback exists)")
+        L = C.CDLL(_LIB)
+        L.libgrpp_last_error.restype = C.c_char_p
+        pp = C.POINTER(dptr)
+        for n in ("libgrpp_type1_integrals", "libgrpp_type2_integrals"):
+            getattr(L, n).argtypes = [PSHELL, PSHELL, dptr, PPOT, dptr]
+        L.libgrpp_spin_orbit_integrals.argtypes = [PSHELL, PSHELL, dptr, PPOT, dptr, dptr, dptr]
+        L.libgrpp_outercore_potential_integrals.argtypes = [PSHELL, PSHELL, dptr, C.c_int, C.POINTER(PPOT),
+                                                            C.POINTER(PSHELL), dptr, dptr, dptr, dptr]
+        L.libgrpp_full_grpp_integrals.argtypes = [PSHELL, PSHELL, PGRPP, dptr, dptr, dptr, dptr, dptr]
+        L.libgrpp_full_grpp_integrals_gradient.argtypes = [PSHELL, PSHELL, PGRPP, dptr, dptr, pp, pp, pp, pp]
+        L.libgrpp_type1_integrals_gradient.argtypes = [PSHELL, PSHELL, dptr, PPOT, dptr, pp]
+        L.libgrpp_type2_integrals_gradient.argtypes = [PSHELL, PSHELL, dptr, PPOT, dptr, pp]
+        L.libgrpp_spin_orbit_integrals_gradient.argtypes = [PSHELL, PSHELL, dptr, PPOT, dptr, pp, pp, pp]
+        L.libgrpp_outercore_potential_integrals_gradient.argtypes = [PSHELL, PSHELL, dptr, C.c_int, C.POINTER(PPOT),
+                                                                     C.POINTER(PSHELL), dptr, pp, pp, pp, pp]
+        L.libgrpp_overlap_integrals.argtypes = [PSHELL, PSHELL, dptr]
+        L.libgrpp_batch_full_grpp_integrals.restype = C.c_int
+        L.libgrpp_batch_full_grpp_integrals.argtypes = [C.c_int, C.POINTER(PSHELL), C.c_int, dptr, C.POINTER(PGRPP),
+                                                        C.c_int, dptr, dptr, dptr, dptr, C.POINTER(Stats)]
+        L.libgrpp_batch_full_grpp_gradients.restype = C.c_int
+        L.libgrpp_batch_full_grpp_gradients.argtypes = [C.c_int, C.POINTER(PSHELL), C.c_int, dptr, C.POINTER(PGRPP),
+                                                        C.c_int, dptr, C.POINTER(Stats)]
+        L.libgrpp_batch_plan_create.restype = C.c_int
+        L.libgrpp_batch_plan_create.argtypes = [C.c_int, C.POINTER(PSHELL), C.c_int, dptr, C.POINTER(PGRPP), C.c_int,
+                                                C.POINTER(C.c_void_p)]
+        L.libgrpp_batch_plan_execute.restype = C.c_int
+        L.libgrpp_batch_plan_execute.argtypes = [C.c_void_p, C.c_void_p]
+        L.libgrpp_batch_plan_download.restype = C.c_int
+        L.libgrpp_batch_plan_download.argtypes = [C.c_void_p, dptr, dptr, dptr, dptr]
+        L.libgrpp_batch_plan_dim.restype = C.c_int
+        L.libgrpp_batch_plan_dim.argtypes = [C.c_void_p]
+        L.libgrpp_batch_plan_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
+        L.libgrpp_batch_plan_destroy.argtypes = [C.c_void_p]
+        L.libgrpp_batch_plan_features.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_float)]
+        L.libgrpp_batch_plan_set_predictions.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float)]
+        L.libgrpp_init()
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().libgrpp_last_error().decode()
+
+
+def factory() -> ObjectFactory:
+    return ObjectFactory(lib(), LIBGRPP_NAMES)
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed ({rc}): {last_error()}")
+
+
+class _Objects:
+    """Native shells and per-centre GRPP objects of a System (freed on close)."""
+
+    def __init__(self, sysm: System):
+        self.f = factory()
+        self.sysm = sysm
+        self.shells = self.f.shells(sysm.shells)
+        nc = len(sysm.centers)
+        self.grpps = (PGRPP * max(1, nc))()
+        self.xyz = np.zeros(3 * max(1, nc))
+        for i, (o, g) in enumerate(sysm.centers):
+            self.grpps[i] = self.f.grpp(g)
+            self.xyz[3 * i:3 * i + 3] = o
+
+    def close(self):
+        self.f.free_shells(self.shells, len(self.sysm.shells))
+        for i in range(len(self.sysm.centers)):
+            self.f.free_grpp(self.grpps[i])
+
+
+def batch_matrices(sysm: System, flags: int = 0):
+    """(arep, so_x, so_y, so_z, stats) for all ordered shell pairs and centres of sysm."""
+    o = _Objects(sysm)
+    dim = sysm.dim
+    out = [np.zeros((dim, dim)) for _ in range(4)]
+    st = Stats()
+    try:
+        rc = lib().libgrpp_batch_full_grpp_integrals(len(sysm.shells), o.shells, len(sysm.centers), as_dptr(o.xyz),
+                                                     o.grpps, flags, *[as_dptr(m) for m in out], C.byref(st))
+        _check(rc, "libgrpp_batch_full_grpp_integrals")
+    finally:
+        o.close()
+    return out[0], out[1], out[2], out[3], st.as_dict()
+
+
+def batch_gradients(sysm: System, flags: int = 0):
+    """grad[op, 3*centre+k] (op = arep, so_x, so_y, so_z) with the centre list as atoms."""
+    o = _Objects(sysm)
+    dim, na = sysm.dim, len(sysm.centers)
+    out = np.zeros((4, 3 * na, dim, dim))
+    st = Stats()
+    try:
+        rc = lib().libgrpp_batch_full_grpp_gradients(len(sysm.shells), o.shells, na, as_dptr(o.xyz), o.grpps, flags,
+                                                     as_dptr(out), C.byref(st))
+        _check(rc, "libgrpp_batch_full_grpp_gradients")
+    finally:
+        o.close()
+    return out, st.as_dict()
+
+
+class Plan:
+    """Packed, device-resident batched build: execute() runs the kernels only."""
+
+    def __init__(self, sysm: System, flags: int = 0):
+        self.sysm = sysm
+        self.o = _Objects(sysm)
+        self.h = C.c_void_p()
+        rc = lib().libgrpp_batch_plan_create(len(sysm.shells), self.o.shells, len(sysm.centers),
+                                             as_dptr(self.o.xyz), self.o.grpps, flags, C.byref(self.h))
+        _check(rc, "libgrpp_batch_plan_create")
+        self.dim = lib().libgrpp_batch_plan_dim(self.h)
+
+    def execute(self, stream: int = 0):
+        _check(lib().libgrpp_batch_plan_execute(self.h, C.c_void_p(stream)), "libgrpp_batch_plan_execute")
+
+    def download(self):
+        out = [np.zeros((self.dim, self.dim)) for _ in range(4)]
+        _check(lib().libgrpp_batch_plan_download(self.h, *[as_dptr(m) for m in out]), "libgrpp_batch_plan_download")
+        return out
+
+    def stats(self):
+        st = Stats()
+        lib().libgrpp_batch_plan_stats(self.h, C.byref(st))
+        return st.as_dict()
+
+    def close(self):
+        if self.h:
+            lib().libgrpp_batch_plan_destroy(self.h)
+            self.h = C.c_void_p()
+        self.o.close()
+
+
+# ---------------------------------------------------------------- per-call API (libgrpp.h:123-231)
+def _grad_arrays(na: int, nb: int):
+    arrs = [np.zeros(na * nb) for _ in range(3)]
+    pp = (dptr * 3)(*[as_dptr(a) for a in arrs])
+    return arrs, pp
+
+
+class Api:
+    """Thin object-level mirror of the reference calls, for tests and examples."""
+
+    def __init__(self):
+        self.f = factory()
+        self.L = lib()
+
+    def type1(self, A: Shell, B: Shell, C: Tuple[float, float, float], pot: Optional[Potential]) -> np.ndarray:
+        a, b = self.f.shell(A), self.f.shell(B)
+        p = self.f.potential(pot) if pot is not None else None
+        m = np.zeros(A.cart_size * B.cart_size)
+        self.L.libgrpp_type1_integrals(a, b, darr(C), p, as_dptr(m))
+        self._free(a, b, p)
+        return m.reshape(A.cart_size, B.cart_size)
+
+    def type2(self, A, B, C, pot):
+        a, b, p = self.f.shell(A), self.f.shell(B), self.f.potential(pot)
+        m = np.zeros(A.cart_size * B.cart_size)
+        self.L.libgrpp_type2_integrals(a, b, darr(C), p, as_dptr(m))
+        self._free(a, b, p)
+        return m.reshape(A.cart_size, B.cart_size)
+
+    def spin_orbit(self, A, B, C, pot):
+        a, b, p = self.f.shell(A), self.f.shell(B), self.f.potential(pot)
+        ms = [np.zeros(A.cart_size * B.cart_size) for _ in range(3)]
+        self.L.libgrpp_spin_orbit_integrals(a, b, darr(C), p, *[as_dptr(m) for m in ms])
+        self._free(a, b, p)
+        return [m.reshape(A.cart_size, B.cart_size) for m in ms]
+
+    def outercore(self, A, B, C, oc: List[Tuple[Potential, Shell]]):
+        a, b = self.f.shell(A), self.f.shell(B)
+        n = len(oc)
+        pots = (PPOT * max(1, n))(*[self.f.potential(p) for p, _ in oc])
+        shs = (PSHELL * max(1, n))(*[self.f.shell(s) for _, s in oc])
+        ms = [np.zeros(A.cart_size * B.cart_size) for _ in range(4)]
+        self.L.libgrpp_outercore_potential_integrals(a, b, darr(C), n, pots, shs, *[as_dptr(m) for m in ms])
+        for i in range(n):
+            self.f.f["delete_potential"](pots[i])
+            self.f.f["delete_shell"](shs[i])
+        self._free(a, b, None)
+        return [m.reshape(A.cart_size, B.cart_size) for m in ms]
+
+    def full(self, A, B, C, g: GRPP):
+        a, b, h = self.f.shell(A), self.f.shell(B), self.f.grpp(g)
+        ms = [np.zeros(A.cart_size * B.cart_size) for _ in range(4)]
+        self.L.libgrpp_full_grpp_integrals(a, b, h, darr(C), *[as_dptr(m) for m in ms])
+        self.f.free_grpp(h)
+        self._free(a, b, None)
+        return [m.reshape(A.cart_size, B.cart_size) for m in ms]
+
+    def full_gradient(self, A, B, C, g: GRPP, D):
+        a, b, h = self.f.shell(A), self.f.shell(B), self.f.grpp(g)
+        na, nb = A.cart_size, B.cart_size
+        outs = [_grad_arrays(na, nb) for _ in range(4)]
+        self.L.libgrpp_full_grpp_integrals_gradient(a, b, h, darr(C), darr(D), *[pp for _, pp in outs])
+        self.f.free_grpp(h)
+        self._free(a, b, None)
+        return np.array([[x.reshape(na, nb) for x in arrs] for arrs, _ in outs])
+
+    def _free(self, a, b, p):
+        self.f.f["delete_shell"](a)
+        self.f.f["delete_shell"](b)
+        if p is not None:
+            self.f.f["delete_potential"](p)
