@@ -330,8 +330,13 @@ __global__ __launch_bounds__(256) void k_radial(const TileItem *__restrict__ til
         if (!__syncthreads_or(done != 0xFFFFu)) break;
     }
     if (tid == 0 && counters) {
+        // algorithmic work: distinct table entries of the tile (upper triangle) x points integrated
+        const int nr = min(RT, jb.M - t.ti * RT), nc = min(RT, jb.M - t.tj * RT);
+        const unsigned long long ent = (t.ti == t.tj) ? (unsigned long long) nr * (nr + 1) / 2
+                                                       : (unsigned long long) nr * nc;
         atomicAdd(&counters[0], (unsigned long long) (32ull << lev_done));   // grid points integrated
         atomicAdd(&counters[1], 1ull);                                       // tiles
+        atomicAdd(&counters[2], ent * (32ull << lev_done));                  // entry x point products
     }
 #pragma unroll
     for (int a = 0; a < 4; a++)
@@ -1103,7 +1108,7 @@ struct eng_plan {
     long long n_tasks = 0, n_integrals = 0, n_entries = 0;
     double flops_fixed = 0.0;    // contraction + type-1 + projector work (grid independent)
     double flops_per_point = 0.0; // radial FLOPs per grid point summed over tiles (x points below)
-    unsigned long long last_points = 0, last_tiles = 0;
+    unsigned long long last_points = 0, last_tiles = 0, last_entry_points = 0;
     double last_seconds = 0.0, last_radial_seconds = 0.0;
     int min_level = 0;
     double tol = 1e-16;
@@ -1527,6 +1532,7 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
     HIPCHK(hipMemcpy(cnt, P->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
     P->last_points = cnt[0];
     P->last_tiles = cnt[1];
+    P->last_entry_points = cnt[2];
     return 0;
 }
 
@@ -1555,8 +1561,8 @@ extern "C" int eng_plan_stats(const eng_plan_t *P, libgrpp_batch_stats_t *s)
     s->n_radial_entries = P->n_entries;
     s->n_radial_points = (long long) P->last_points;
     s->n_tiles = (long long) P->last_tiles;
-    // radial product: 2 FLOP per (entry, point) over the 64x64 tiles actually integrated
-    s->fp64_flops = 2.0 * RT * RT * (double) P->last_points;
+    // radial quadrature: 2 FLOP (one FMA) per distinct (entry, grid point) integrated
+    s->fp64_flops = 2.0 * (double) P->last_entry_points;
     s->device_seconds = P->last_seconds;
     s->radial_seconds = P->last_radial_seconds;
     return 0;

@@ -141,7 +141,7 @@ void orc_full_grpp_gradient(const orc_shell_t *A, const orc_shell_t *B, const or
 /* ---- whole-matrix builders (OpenMP over ordered shell pairs) ---- */
 double orc_build_matrices(int nshell, orc_shell_t **shells, int ncenter, const double *xyz, orc_grpp_t **grpps,
                           double *arep, double *so_x, double *so_y, double *so_z, int nthreads,
-                          int pair_begin, int pair_end);
+                          int npsel, const int *pair_list);
 double orc_build_gradients(int nshell, orc_shell_t **shells, int natom, const double *xyz, orc_grpp_t **grpps,
                            double *out, int nthreads);
 

@@ -135,10 +135,10 @@ static void add_block(int dim, double *m, int r0, int c0, int nr, int nc, const 
 
 /* Dense dim x dim matrices summed over GRPP centres for every ordered shell pair, as the
  * reference C driver assembles them (test_libgrpp_c/eval_integrals.c:42-130).
- * Pairs [pair_begin, pair_end) in row-major pair order; returns seconds in the loop. */
+ * pair_list (npsel entries, p = i*nshell + j) selects ordered pairs, NULL = all; returns seconds in the loop. */
 double orc_build_matrices(int nshell, orc_shell_t **shells, int ncenter, const double *xyz, orc_grpp_t **grpps,
-                          double *arep, double *so_x, double *so_y, double *so_z, int nthreads, int pair_begin,
-                          int pair_end)
+                          double *arep, double *so_x, double *so_y, double *so_z, int nthreads, int npsel,
+                          const int *pair_list)
 {
     orc_init();
     int *off = malloc(sizeof(int) * (nshell + 1));
@@ -150,13 +150,12 @@ double orc_build_matrices(int nshell, orc_shell_t **shells, int ncenter, const d
     memset(so_x, 0, n2 * sizeof(double));
     memset(so_y, 0, n2 * sizeof(double));
     memset(so_z, 0, n2 * sizeof(double));
-    int np = nshell * nshell;
-    if (pair_end < 0 || pair_end > np) pair_end = np;
-    if (pair_begin < 0) pair_begin = 0;
+    int npairs = pair_list ? npsel : nshell * nshell;
     if (nthreads > 0) omp_set_num_threads(nthreads);
     double t0 = omp_get_wtime();
 #pragma omp parallel for schedule(dynamic)
-    for (int p = pair_begin; p < pair_end; p++) {
+    for (int q = 0; q < npairs; q++) {
+        int p = pair_list ? pair_list[q] : q;
         int i = p / nshell, j = p % nshell;
         orc_shell_t *a = shells[i], *b = shells[j];
         int sz = a->cart_size * b->cart_size;

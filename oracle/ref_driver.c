@@ -35,7 +35,7 @@ static void add_block(int dim, double *m, int r0, int c0, int nr, int nc, const 
 /* Returns wall seconds spent in the pair loop (steady clock via omp_get_wtime). */
 double ref_build_grpp_matrices(int nshell, libgrpp_shell_t **shells, int ncenter, const double *xyz,
                                libgrpp_grpp_t **grpps, double *arep, double *so_x, double *so_y, double *so_z,
-                               int nthreads, int pair_begin, int pair_end)
+                               int nthreads, int npsel, const int *pair_list)
 {
     int dim;
     int *off = offsets_of(nshell, shells, &dim);
@@ -44,14 +44,13 @@ double ref_build_grpp_matrices(int nshell, libgrpp_shell_t **shells, int ncenter
     memset(so_x, 0, n2 * sizeof(double));
     memset(so_y, 0, n2 * sizeof(double));
     memset(so_z, 0, n2 * sizeof(double));
-    int npairs = nshell * nshell;
-    if (pair_end < 0 || pair_end > npairs) pair_end = npairs;
-    if (pair_begin < 0) pair_begin = 0;
+    int npairs = pair_list ? npsel : nshell * nshell;
     if (nthreads > 0) omp_set_num_threads(nthreads);
 
     double t0 = omp_get_wtime();
 #pragma omp parallel for schedule(dynamic)
-    for (int p = pair_begin; p < pair_end; p++) {
+    for (int q = 0; q < npairs; q++) {
+        int p = pair_list ? pair_list[q] : q;
         int i = p / nshell, j = p % nshell;
         libgrpp_shell_t *a = shells[i], *b = shells[j];
         int sz = a->cart_size * b->cart_size;

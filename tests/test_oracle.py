@@ -1,0 +1,90 @@
+"""The oracle (CPU restatement of libgrpp's GRPP path) against the reference's own data:
+golden matrices and gradients held by the reference's tests, the genuine library compiled from
+its sources (oracle/_ref, when present), and the seeded small_pair reference vectors."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, golden_matrix
+from libgrpp_amd import configs, io
+
+OPS = ["arep", "so_x", "so_y", "so_z"]
+
+
+@pytest.mark.parametrize("case", ["H2O", "Cs"])
+def test_oracle_matrices_match_reference_goldens(orc, case, request):
+    sysm = request.getfixturevalue(case.lower())
+    M = orc.matrices(sysm)
+    for k, nm in enumerate(OPS):
+        err, rel, ok = io.parity_error(np.tril(M[k]), golden_matrix(case, nm))
+        assert ok, (case, nm, err, rel)
+
+
+def test_oracle_gradients_match_reference_goldens(orc, h2o):
+    """all 36 gradient matrices of test/H2O (3 atoms x xyz x arep/so_x/so_y/so_z, outercore included)"""
+    G, _ = orc.gradients(h2o)
+    for op, nm in enumerate(OPS):
+        for a in range(3):
+            for k, c in enumerate("xyz"):
+                err, rel, ok = io.parity_error(np.tril(G[op, 3 * a + k]), golden_matrix("H2O", f"{nm}_grad_{a}{c}"))
+                assert ok, (nm, a, c, err)
+
+
+def test_oracle_small_pair_matches_reference_vectors(orc):
+    """seeded small_pair workload: type-1, type-2, SO and full blocks of the genuine libgrpp"""
+    d = np.load(os.path.join(GOLDEN, "small_pair_reference.npz"))
+    sysm = configs.small_pair()
+    M = orc.matrices(sysm)
+    for k, nm in enumerate(OPS):
+        assert io.parity_error(M[k], d[nm])[2], nm
+    origin, g = sysm.centers[0]
+    off = sysm.offsets()
+    T1, T2, SO = np.zeros_like(d["type1"]), np.zeros_like(d["type2"]), np.zeros_like(d["so"])
+    for i, A in enumerate(sysm.shells):
+        for j, B in enumerate(sysm.shells):
+            sl = (slice(off[i], off[i] + A.cart_size), slice(off[j], off[j] + B.cart_size))
+            T1[sl] = orc.per_call("type1", A, B, origin, g.local)
+            for p in g.arep:
+                T2[sl] += orc.per_call("type2", A, B, origin, p)
+            for p in g.esop:
+                SO[(slice(None),) + sl] += 2.0 / (2 * p.L + 1) * orc.per_call("so", A, B, origin, p)
+    assert io.parity_error(T1, d["type1"])[2]
+    assert io.parity_error(T2, d["type2"])[2]
+    for k in range(3):
+        assert io.parity_error(SO[k], d["so"][k])[2]
+
+
+def test_oracle_gradient_branches_match_reference(orc, h2o):
+    """libgrpp_full_grpp_integrals_gradient per task over the ACB / ACA / ACC / CCB / AAA branches"""
+    d = np.load(os.path.join(GOLDEN, "h2o_pergrad_reference.npz"))
+    atoms = [c for c, _ in h2o.centers]
+    pos = 0
+    for (a, b, c, dd), n in zip(d["tasks"], d["lengths"]):
+        got = orc.per_call("grad", h2o.shells[a], h2o.shells[b], atoms[c], h2o.centers[c][1], D=atoms[dd]).ravel()
+        want = d["values"][pos:pos + n]
+        pos += n
+        assert io.parity_error(got, want)[2], (a, b, c, dd)
+
+
+def test_oracle_matches_compiled_reference_per_call(orc, ref, h2o):
+    """every per-call integrator, oracle vs the genuine library, on shell pairs of H2O"""
+    pairs = [(0, 14, 2), (3, 19, 1), (10, 11, 0), (16, 17, 1), (13, 21, 2)]
+    for a, b, c in pairs:
+        A, B = h2o.shells[a], h2o.shells[b]
+        C, g = h2o.centers[c]
+        for what, arg in [("type1", g.local), ("type2", g.arep[0]), ("so", g.esop[0]), ("oc", g.oc), ("full", g)]:
+            x, y = orc.per_call(what, A, B, C, arg), ref.per_call(what, A, B, C, arg)
+            assert io.parity_error(x, y)[2], (what, a, b, c)
+
+
+def test_oracle_matches_compiled_reference_heavy_sample(orc, ref):
+    """random ordered pairs of the heavy_cluster workload (l up to 4, SO, outercore)"""
+    sysm = configs.heavy_cluster(natom=3)
+    ns = len(sysm.shells)
+    pairs = np.sort(np.random.default_rng(3).permutation(ns * ns)[:24])
+    X = orc.matrices(sysm, pairs=pairs)
+    Y = ref.matrices(sysm, pairs=pairs)
+    for k in range(4):
+        err, rel, ok = io.parity_error(X[k], Y[k])
+        assert ok, (OPS[k], err, rel)
