@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -90,19 +91,22 @@ struct TabItem {
     int c, A, frow, lam;
 };
 struct RowDesc {
-    int frow, n;
+    int frow, n, shell, lam;
 };
 struct QJob {
     int rows, M, pot, maxlev;
     long long qoff;
     double thr;
+    int center, potential;   // centre and potential record (screening)
+    int L, pad;
 };
 struct TileItem {
     int job, ti, tj, pad;
 };
 struct ProjItem {
     int c, A, L, K;
-    long long poff, ploff;
+    long long poff, ploff;   // ket projector, ket L-products (SO)
+    long long pb2, pb3;      // bra projectors with |Omega| < 1e-16 (type-2) / < 1e-14 (SO) dropped
 };
 struct OcvItem {
     int c, A, oc, pad;      // oc: global OC index (potential + shell)
@@ -130,6 +134,8 @@ struct DevJob {
     const int *slot_rowoff;        // [slot * nshell + A]
     const long long *slot_poff;    // [slot * nshell + A]
     const long long *slot_ploff;   // [slot * nshell + A] (-1 if the slot has no SO channel)
+    const long long *slot_pb2;     // bra projector, type-2 thresholds
+    const long long *slot_pb3;     // bra projector, spin-orbit thresholds
     const long long *ocv_tab;      // [oc * nshell + A]
     const double *delta;
     const double *Q, *P, *OCV;
@@ -141,6 +147,32 @@ struct DevJob {
 __device__ __forceinline__ int dev_ncart(int l) { return (l + 1) * (l + 2) / 2; }
 __device__ __forceinline__ int dev_lam_lo(int L, int n) { return (n <= L) ? (L - n) : ((L + n) & 1); }
 __device__ __forceinline__ int dev_lam_cnt(int L, int n) { return (L + n - dev_lam_lo(L, n)) / 2 + 1; }
+__device__ __forceinline__ int dev_colbase(int L, int n)
+{
+    int k = 0;
+    for (int q = 0; q < n; q++) k += dev_lam_cnt(L, q);
+    return k;
+}
+__device__ __forceinline__ bool dev_valid(int L, int n, int lam)
+{
+    return n >= 0 && lam >= dev_lam_lo(L, n) && lam <= L + n && ((L + n - lam) & 1) == 0;
+}
+// column of (n, lambda) in a shell's projector block
+__device__ __forceinline__ int dev_kcol(int L, int n, int lam) { return dev_colbase(L, n) + (lam - dev_lam_lo(L, n)) / 2; }
+// libgrpp keeps one radial integral per (lambda1, lambda2, N = nA + nB); the canonical split is the
+// smallest valid nA, so every split of the same integral reads the same number
+__device__ __forceinline__ int dev_canon_nA(int L, int la, int lb, int lam1, int lam2, int N)
+{
+    for (int n = max(0, N - lb); n <= min(la, N); n++)
+        if (dev_valid(L, n, lam1) && dev_valid(L, N - n, lam2)) return n;
+    return -1;
+}
+__device__ __forceinline__ int dev_canon_nA_max(int L, int la, int lb, int lam1, int lam2, int N)
+{
+    for (int n = min(la, N); n >= max(0, N - lb); n--)
+        if (dev_valid(L, n, lam1) && dev_valid(L, N - n, lam2)) return n;
+    return -1;
+}
 __device__ __forceinline__ int dev_kcols(int L, int l)
 {
     int k = 0;
@@ -173,6 +205,21 @@ __device__ __forceinline__ double dev_binom(int n, int k)
     return r;
 }
 __device__ __forceinline__ double gnorm(int L, double a) { return pow(2 * a / M_PI, 0.75) * pow(4 * a, 0.5 * L); }
+
+// int_0^inf r^n exp(-a r^2) dr (screening.c:255-265)
+__device__ __forceinline__ double int_gauss_s(int n, double a)
+{
+    if (n % 2 == 0) {
+        const int k = n / 2;
+        double df = 1.0;
+        for (int z = 2 * k - 1; z > 1; z -= 2) df *= z;
+        return df / (pow(2.0, k + 1) * pow(a, k)) * sqrt(M_PI / a);
+    }
+    const int k = (n - 1) / 2;
+    double fk = 1.0;
+    for (int z = 2; z <= k; z++) fk *= z;
+    return fk / (2.0 * pow(a, k + 1));
+}
 
 // ================================================================== tabulation kernels
 constexpr int MAXLAM = LMAX_POT + LMAX_SHELL;   // 20
@@ -236,15 +283,182 @@ __global__ __launch_bounds__(256) void k_tab_wU(const int *__restrict__ pots, De
     wU[(size_t) blockIdx.y * NPTS + i] = T.w[i] * v;
 }
 
-// ================================================================== radial quadrature as a tiled product
-constexpr int RT = 64;   // tile edge (entries)
+constexpr int RT = 64;   // radial tile edge (entries)
 constexpr int RK = 32;   // grid points per K step
+
+// ================================================================== Shaw-Hill screening of the radial entries
+// flat index of the canonical split of entry (row, col) of a radial table
+__device__ __forceinline__ long long canon_index(const RowDesc *R, const DShell *sh, const QJob &jb, int row, int col)
+{
+    const RowDesc ra = R[row], rb = R[col];
+    const int L = jb.L, la = sh[ra.shell].L, lb = sh[rb.shell].L, N = ra.n + rb.n;
+    const bool a_first = (ra.shell < rb.shell) || (ra.shell == rb.shell && ra.lam <= rb.lam);
+    const int cn = a_first ? dev_canon_nA(L, la, lb, ra.lam, rb.lam, N) : dev_canon_nA_max(L, la, lb, ra.lam, rb.lam, N);
+    const int r2 = row - dev_kcol(L, ra.n, ra.lam) + dev_kcol(L, cn, ra.lam);
+    const int c2 = col - dev_kcol(L, rb.n, rb.lam) + dev_kcol(L, N - cn, rb.lam);
+    return (long long) r2 * jb.M + c2;
+}
+
+
+// Envelope estimate of one radial integral from its primitive triples (screening.c:193-378):
+// the integrand maximum r0 is found by <= 4 fixed-point steps of tolerance 1e-2 using Bessel ratios;
+// any triple that does not settle makes the screening inert.  Inert screening is encoded as NaN,
+// which the quadrature's tests treat exactly as the reference treats a failed screening.
+// K_{l-1}(x)/K_l(x) (l > 0) or K_1(x)/K_0(x) (l = 0): both orders from one table node
+__device__ __forceinline__ double bessel_ratio(const DevTables &T, int l, double x)
+{
+    const int a = (l == 0) ? 1 : l - 1, b = (l == 0) ? 0 : l;
+    if (x > 1e-7 && x <= 16.0) {
+        const int i0 = (int) (rint(x * 100) + 1e-5);
+        const double dx = x - i0 * 0.01, dx2 = dx * dx, dx3 = dx2 * dx, dx4 = dx3 * dx;
+        const double *da = T.bes + ((size_t) i0 * BES_NL + a) * 5, *db = T.bes + ((size_t) i0 * BES_NL + b) * 5;
+        const double ka = da[0] + da[1] * dx + da[2] * dx2 * 0.5 + da[3] * dx3 * (1.0 / 6.0) + da[4] * dx4 * (1.0 / 24.0);
+        const double kb = db[0] + db[1] * dx + db[2] * dx2 * 0.5 + db[3] * dx3 * (1.0 / 6.0) + db[4] * dx4 * (1.0 / 24.0);
+        return ka / kb;
+    }
+    return bessel_k(T.bes, T.rfun, a, x) / bessel_k(T.bes, T.rfun, b, x);
+}
+
+__device__ double screen_fixed_point_eq(const DevTables &T, double r, int n, int l1, int l2, double p, double k1,
+                                        double k2)
+{
+    const double k1r = k1 * r, k2r = k2 * r;
+    const double K1 = bessel_ratio(T, l1, k1r);
+    const double K2 = bessel_ratio(T, l2, k2r);
+    double a = K1 * k1r + K2 * k2r + n;
+    if (l1 > 0) a = a - l1 - 1;
+    if (l2 > 0) a = a - l2 - 1;
+    return sqrt(a / (2.0 * p));
+}
+
+__device__ double screen_entry(const DevJob &J, const DevTables &T, const DShell &A, const DShell &B, double CA,
+                               double CB, const DPot &P, int l1, int l2, int N)
+{
+    const double CA2 = CA * CA, CB2 = CB * CB;
+    double est = 0.0;
+    for (int i = 0; i < A.nprim; i++) {
+        const double ci = J.cnorm[A.poff + i];
+        if (ci == 0.0) continue;                     // primitives dropped by libgrpp_shell_shrink
+        const double aA = J.alpha[A.poff + i], k1 = 2 * aA * CA;
+        for (int j = 0; j < B.nprim; j++) {
+            const double cj = J.cnorm[B.poff + j];
+            if (cj == 0.0) continue;
+            const double aB = J.alpha[B.poff + j], k2 = 2 * aB * CB;
+            for (int t = 0; t < P.nterm; t++) {
+                const double eta = J.tzeta[P.toff + t];
+                const int n = N + J.tpow[P.toff + t];
+                double val = 0.0;
+                if ((l1 >= 1 && fabs(k1) <= 1e-14) || (l2 >= 1 && fabs(k2) <= 1e-14)) {
+                    val = 0.0;
+                } else {
+                    const double p = aA + aB + eta;
+                    if (l1 == 0 && l2 == 0 && fabs(k1) <= 1e-14 && fabs(k2) <= 1e-14) {
+                        val = exp(-aA * CA * CA - aB * CB * CB) * int_gauss_s(n, p);
+                    } else {
+                        double r0 = (aA * CA + aB * CB) / p, r0p;
+                        int steps = 0;
+                        bool fail = false;
+                        do {
+                            steps++;
+                            if (steps == 5) { fail = true; break; }
+                            r0p = r0;
+                            r0 = screen_fixed_point_eq(T, r0, n, l1, l2, p, k1, k2);
+                        } while (fabs(r0 - r0p) > 1e-2);
+                        if (fail) return __longlong_as_double(0x7ff8000000000000LL);
+                        val = sqrt(M_PI / p) * pow(r0, n) * bessel_k(T.bes, T.rfun, l1, k1 * r0) *
+                              bessel_k(T.bes, T.rfun, l2, k2 * r0) *
+                              exp(-eta * r0 * r0 - aA * (r0 - CA) * (r0 - CA) - aB * (r0 - CB) * (r0 - CB)) * 0.5 *
+                              (1 + erf(sqrt(p) * r0));
+                    }
+                }
+                est += ci * cj * J.tcoef[P.toff + t] * val;
+            }
+        }
+    }
+    (void) CA2;
+    (void) CB2;
+    return est;
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_screen(const TileItem *__restrict__ tiles, const QJob *__restrict__ jobs,
+                                                const RowDesc *__restrict__ rows, DevJob J, DevTables T,
+                                                double *__restrict__ scr)
+{
+    const TileItem t = tiles[blockIdx.x];
+    const QJob jb = jobs[t.job];
+    const RowDesc *R = rows + jb.rows;
+    const DCenter c = J.ctr[jb.center];
+    const DPot P = J.pot[jb.potential];
+    for (int e = threadIdx.x; e < RT * RT; e += blockDim.x) {
+        const int row = t.ti * RT + e / RT, col = t.tj * RT + e % RT;
+        if (row >= jb.M || col >= jb.M) continue;
+        if (t.ti == t.tj && row > col) continue;
+        RowDesc ra = R[row], rb = R[col];
+        {
+            // only the canonical split of (lambda1, lambda2, N) is screened; the others read it
+            const int la = J.sh[ra.shell].L, lb = J.sh[rb.shell].L, N = ra.n + rb.n;
+            const bool a_first = (ra.shell < rb.shell) || (ra.shell == rb.shell && ra.lam <= rb.lam);
+            const int cn = a_first ? dev_canon_nA(jb.L, la, lb, ra.lam, rb.lam, N)
+                                   : dev_canon_nA_max(jb.L, la, lb, ra.lam, rb.lam, N);
+            if (cn != ra.n) continue;
+        }
+        // a fixed orientation of the (A, B, lambda1, lambda2, N) tuple so that every split and both
+        // triangles get bitwise the same estimate
+        if (ra.shell > rb.shell || (ra.shell == rb.shell && ra.lam > rb.lam)) {
+            const RowDesc tmp = ra;
+            ra = rb;
+            rb = tmp;
+        }
+        const DShell A = J.sh[ra.shell], B = J.sh[rb.shell];
+        const double ax = c.x - A.x, ay = c.y - A.y, az = c.z - A.z, bx = c.x - B.x, by = c.y - B.y, bz = c.z - B.z;
+        const double CA = sqrt(ax * ax + ay * ay + az * az), CB = sqrt(bx * bx + by * by + bz * bz);
+        const double v = screen_entry(J, T, A, B, CA, CB, P, ra.lam, rb.lam, ra.n + rb.n);
+        scr[jb.qoff + (long long) row * jb.M + col] = v;
+        scr[jb.qoff + (long long) col * jb.M + row] = v;
+    }
+}
+
+// libgrpp keeps ONE radial integral per (lambda1, lambda2, N = nA + nB) and uses it for every split
+// (type2_integrals.c:183-186); the product above evaluates each split separately.  This pass makes
+// every split read one canonical split (smallest n on the side of the smaller (shell, lambda) key,
+// which keeps the table symmetric), so rounding differences between splits cannot leak into the
+// cancellation of the binomial expansion.
+__global__ __launch_bounds__(256) void k_canon(const TileItem *__restrict__ tiles, const QJob *__restrict__ jobs,
+                                               const RowDesc *__restrict__ rows, DevJob J,
+                                               const double *__restrict__ Q, double *__restrict__ Qc)
+{
+    const TileItem t = tiles[blockIdx.x];
+    const QJob jb = jobs[t.job];
+    const RowDesc *R = rows + jb.rows;
+    const int L = jb.L;
+    for (int e = threadIdx.x; e < 2 * RT * RT; e += blockDim.x) {
+        const int half = e / (RT * RT), q = e % (RT * RT);
+        int row = t.ti * RT + q / RT, col = t.tj * RT + q % RT;
+        if (half) {
+            if (t.ti == t.tj) continue;
+            const int tmp = row;
+            row = col;
+            col = tmp;
+        }
+        if (row >= jb.M || col >= jb.M) continue;
+        const RowDesc ra = R[row], rb = R[col];
+        const int la = J.sh[ra.shell].L, lb = J.sh[rb.shell].L, N = ra.n + rb.n;
+        const bool a_first = (ra.shell < rb.shell) || (ra.shell == rb.shell && ra.lam <= rb.lam);
+        const int cn = a_first ? dev_canon_nA(L, la, lb, ra.lam, rb.lam, N) : dev_canon_nA_max(L, la, lb, ra.lam, rb.lam, N);
+        const int r2 = row - dev_kcol(L, ra.n, ra.lam) + dev_kcol(L, cn, ra.lam);
+        const int c2 = col - dev_kcol(L, rb.n, rb.lam) + dev_kcol(L, N - cn, rb.lam);
+        Qc[jb.qoff + (long long) row * jb.M + col] = Q[jb.qoff + (long long) r2 * jb.M + c2];
+    }
+}
+
+// ================================================================== radial quadrature as a tiled product
 
 __global__ __launch_bounds__(256) void k_radial(const TileItem *__restrict__ tiles, const QJob *__restrict__ jobs,
                                                 const RowDesc *__restrict__ rows, const double *__restrict__ F,
                                                 const double *__restrict__ rpow, const double *__restrict__ wU,
                                                 double *__restrict__ Q, int min_level, double tol,
-                                                unsigned long long *__restrict__ counters)
+                                                unsigned long long *__restrict__ counters,
+                                                const double *__restrict__ scr, const DShell *__restrict__ J_sh)
 {
     __shared__ double As[RK][RT + 2];
     __shared__ double Bs[RK][RT + 2];
@@ -266,7 +480,7 @@ __global__ __launch_bounds__(256) void k_radial(const TileItem *__restrict__ til
     const int tx = tid & 15, ty = tid >> 4;
     const double *wp = wU + (size_t) jb.pot * NPTS;
 
-    double acc[4][4], S[4][4], res[4][4];
+    double acc[4][4], S[4][4], res[4][4], est[4][4];
     unsigned done = 0;
 #pragma unroll
     for (int a = 0; a < 4; a++)
@@ -274,8 +488,18 @@ __global__ __launch_bounds__(256) void k_radial(const TileItem *__restrict__ til
         for (int b = 0; b < 4; b++) {
             S[a][b] = 0.0;
             res[a][b] = 0.0;
+            est[a][b] = __longlong_as_double(0x7ff8000000000000LL);
             const int row = t.ti * RT + ty * 4 + a, col = t.tj * RT + tx * 4 + b;
-            if (row >= jb.M || col >= jb.M) done |= 1u << (a * 4 + b);
+            if (row >= jb.M || col >= jb.M) {
+                done |= 1u << (a * 4 + b);
+            } else if (scr) {
+                // screened below the tolerance: the envelope estimate is the result (radial_type2_integral.c:277-280)
+                est[a][b] = scr[jb.qoff + canon_index(R, J_sh, jb, row, col)];
+                if (!isnan(est[a][b]) && fabs(est[a][b]) < tol) {
+                    done |= 1u << (a * 4 + b);
+                    res[a][b] = est[a][b];
+                }
+            }
         }
 
     int lev_done = 0;
@@ -320,7 +544,9 @@ __global__ __launch_bounds__(256) void k_radial(const TileItem *__restrict__ til
                 } else {
                     const double prev = S[a][b];
                     S[a][b] = 0.5 * S[a][b] + acc[a][b];
-                    if (!(done & bit) && lev >= min_level && fabs(S[a][b] - prev) <= tol) {
+                    // no stop while |S| < 1e-3 |estimate| (radial_type2_integral.c:308-311)
+                    const bool forced = !isnan(est[a][b]) && (fabs(S[a][b]) / fabs(est[a][b]) < 0.001);
+                    if (!(done & bit) && !forced && lev >= min_level && fabs(S[a][b] - prev) <= tol) {
                         done |= bit;
                         res[a][b] = S[a][b];
                     }
@@ -352,6 +578,141 @@ __global__ __launch_bounds__(256) void k_radial(const TileItem *__restrict__ til
                 Q[jb.qoff + (long long) col * jb.M + row] = v;
             }
         }
+}
+
+// The same quadrature on the FP64 matrix cores: each wave owns a 32x32 quarter of the tile as
+// 2x2 blocks of v_mfma_f64_16x16x4f64 (A[i][k] in lane i+16k, B[k][j] in lane j+16k, D[l/16+4r][l%16]
+// in element r of lane l, as measured on gfx950); the per-entry stop rule is unchanged.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_radial_mfma(const TileItem *__restrict__ tiles, const QJob *__restrict__ jobs,
+                                                     const RowDesc *__restrict__ rows, const double *__restrict__ F,
+                                                     const double *__restrict__ rpow, const double *__restrict__ wU,
+                                                     double *__restrict__ Q, int min_level, double tol,
+                                                     unsigned long long *__restrict__ counters,
+                                                     const double *__restrict__ scr, const DShell *__restrict__ J_sh)
+{
+    __shared__ double As[RK][RT + 2];
+    __shared__ double Bs[RK][RT + 2];
+    __shared__ int sfa[RT], sna[RT], sfb[RT], snb[RT];
+    const TileItem t = tiles[blockIdx.x];
+    const QJob jb = jobs[t.job];
+    const int tid = threadIdx.x;
+    const RowDesc *R = rows + jb.rows;
+    if (tid < RT) {
+        int r = t.ti * RT + tid;
+        sfa[tid] = (r < jb.M) ? R[r].frow : -1;
+        sna[tid] = (r < jb.M) ? R[r].n : 0;
+    } else if (tid < 2 * RT) {
+        int r = t.tj * RT + tid - RT;
+        sfb[tid - RT] = (r < jb.M) ? R[r].frow : -1;
+        snb[tid - RT] = (r < jb.M) ? R[r].n : 0;
+    }
+    __syncthreads();
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    const double *wp = wU + (size_t) jb.pot * NPTS;
+
+    v4d acc[2][2];
+    double S[2][2][4], res[2][2][4], est[2][2][4];
+    unsigned done = 0;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                S[a][b][r] = 0.0;
+                res[a][b][r] = 0.0;
+                est[a][b][r] = __longlong_as_double(0x7ff8000000000000LL);
+                const int row = t.ti * RT + wr + a * 16 + lk + 4 * r, col = t.tj * RT + wc + b * 16 + li;
+                if (row >= jb.M || col >= jb.M) {
+                    done |= 1u << (a * 8 + b * 4 + r);
+                } else if (scr) {
+                    est[a][b][r] = scr[jb.qoff + canon_index(R, J_sh, jb, row, col)];
+                    if (!isnan(est[a][b][r]) && fabs(est[a][b][r]) < tol) {
+                        done |= 1u << (a * 8 + b * 4 + r);
+                        res[a][b][r] = est[a][b][r];
+                    }
+                }
+            }
+
+    int lev_done = 0;
+    for (int lev = 0; lev <= jb.maxlev; lev++) {
+        const int kb = (lev == 0) ? 0 : (16 << lev);
+        const int ke = (lev == 0) ? 32 : (32 << lev);
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+        for (int k0 = kb; k0 < ke; k0 += RK) {
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int e = tid + 256 * q, r = e >> 5, kk = e & 31, k = k0 + kk;
+                const int fa = sfa[r], fb = sfb[r];
+                As[kk][r] = (fa >= 0) ? F[(size_t) fa * NPTS + k] * rpow[sna[r] * NPTS + k] : 0.0;
+                Bs[kk][r] = (fb >= 0) ? F[(size_t) fb * NPTS + k] * rpow[snb[r] * NPTS + k] * wp[k] : 0.0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < RK; kk += 4) {
+                const double a0 = As[kk + lk][wr + li], a1 = As[kk + lk][wr + 16 + li];
+                const double b0 = Bs[kk + lk][wc + li], b1 = Bs[kk + lk][wc + 16 + li];
+                acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+                acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+                acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+                acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const unsigned bit = 1u << (a * 8 + b * 4 + r);
+                    const double v = acc[a][b][r];
+                    if (lev == 0) {
+                        S[a][b][r] = v;
+                    } else {
+                        const double prev = S[a][b][r];
+                        S[a][b][r] = 0.5 * S[a][b][r] + v;
+                        const bool forced = !isnan(est[a][b][r]) && (fabs(S[a][b][r]) / fabs(est[a][b][r]) < 0.001);
+                        if (!(done & bit) && !forced && lev >= min_level && fabs(S[a][b][r] - prev) <= tol) {
+                            done |= bit;
+                            res[a][b][r] = S[a][b][r];
+                        }
+                    }
+                }
+        lev_done = lev;
+        if (!__syncthreads_or(done != 0xFFFFu)) break;
+    }
+    if (tid == 0 && counters) {
+        const int nr = min(RT, jb.M - t.ti * RT), nc = min(RT, jb.M - t.tj * RT);
+        const unsigned long long ent = (t.ti == t.tj) ? (unsigned long long) nr * (nr + 1) / 2
+                                                       : (unsigned long long) nr * nc;
+        atomicAdd(&counters[0], (unsigned long long) (32ull << lev_done));
+        atomicAdd(&counters[1], 1ull);
+        atomicAdd(&counters[2], ent * (32ull << lev_done));
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const unsigned bit = 1u << (a * 8 + b * 4 + r);
+                const int row = t.ti * RT + wr + a * 16 + lk + 4 * r, col = t.tj * RT + wc + b * 16 + li;
+                if (row >= jb.M || col >= jb.M) continue;
+                double v = (done & bit) ? res[a][b][r] : S[a][b][r];
+                if (fabs(v) < jb.thr) v = 0.0;
+                if (t.ti < t.tj || row <= col) {
+                    Q[jb.qoff + (long long) row * jb.M + col] = v;
+                    Q[jb.qoff + (long long) col * jb.M + row] = v;
+                }
+            }
 }
 
 // ================================================================== angular projectors
@@ -418,13 +779,15 @@ __global__ __launch_bounds__(256) void k_proj(const ProjItem *__restrict__ items
             Om[e] = o;
         }
         __syncthreads();
+        // three projectors: ket (all Omega), bra with the type-2 skip |Omega| < 1e-16
+        // (type2_integrals.c:232-234) and bra with the SO skip |Omega| < 1e-14 (spin_orbit_integrals.c:275-277)
         // P[i][m][colbase + j] = sum_{abc <= cart_i, a+b+c = n} binomials * CA powers * Omega
         const int np = nc * nm * cnt;
         for (int e = tid; e < np; e += blockDim.x) {
             const int j = e % cnt, im = e / cnt, m = im % nm, i = im / nm;
             int cx, cy, cz;
             dev_cart(la, i, cx, cy, cz);
-            double v = 0.0;
+            double v = 0.0, v2 = 0.0, v3 = 0.0;
             for (int a = 0; a <= cx && a <= n; a++) {
                 const double fa = dev_binom(cx, a) * pow(CAx, cx - a);
                 for (int b = 0; b <= cy && a + b <= n; b++) {
@@ -432,10 +795,16 @@ __global__ __launch_bounds__(256) void k_proj(const ProjItem *__restrict__ items
                     if (cc > cz) continue;
                     const double f = fa * dev_binom(cy, b) * pow(CAy, cy - b) * dev_binom(cz, cc) * pow(CAz, cz - cc);
                     const int q = (n - a) * (n - a + 1) / 2 + (n - a - b);
-                    v += f * Om[(j * nm + m) * nab + q];
+                    const double o = Om[(j * nm + m) * nab + q];
+                    v += f * o;
+                    if (fabs(o) >= 1e-16) v2 += f * o;
+                    if (fabs(o) >= 1e-14) v3 += f * o;
                 }
             }
-            P[it.poff + ((long long) i * nm + m) * K + colbase + j] = v;
+            const long long idx = ((long long) i * nm + m) * K + colbase + j;
+            P[it.poff + idx] = v;
+            P[it.pb2 + idx] = v2;
+            P[it.pb3 + idx] = v3;
         }
         colbase += cnt;
         __syncthreads();
@@ -795,19 +1164,22 @@ __device__ __forceinline__ double int_gauss(int n, double a)
     return fk / (2.0 * pow(a, k + 1));
 }
 
+constexpr int TS_MAX = 6144;   // doubles of LDS for the (m, i, k2) intermediate
+
 __global__ __launch_bounds__(256) void k_contract(const WorkItem *__restrict__ items, DevJob J, DevTables T,
-                                                  double *__restrict__ out, int parts)
+                                                  double *__restrict__ out, int parts, int nm_max)
 {
     __shared__ double Qs[66 * 66];
-    __shared__ double Ts[66 * 66];
+    __shared__ double Ts[TS_MAX];
     const WorkItem it = items[blockIdx.x];
     const DShell sa = J.sh[it.a], sb = J.sh[it.b];
     const int la = sa.L, lb = sb.L, ncA = dev_ncart(la), ncB = dev_ncart(lb);
     const int tid = threadIdx.x;
     const double FOURPI2 = 16.0 * M_PI * M_PI;
-    const int rows_per_chunk = max(1, (CT_EPT * 256) / ncB);
+    // rows of A per pass: the outputs fit CT_EPT per thread and T[m][i][k2] fits TS_MAX
+    const int rows_per_chunk = max(1, min((CT_EPT * 256) / ncB, TS_MAX / (nm_max * ncB)));
     for (int r0 = 0; r0 < ncA; r0 += rows_per_chunk) {
-        const int r1 = min(ncA, r0 + rows_per_chunk), nel = (r1 - r0) * ncB;
+        const int r1 = min(ncA, r0 + rows_per_chunk), nr = r1 - r0, nel = nr * ncB;
         double acc[4][CT_EPT];
 #pragma unroll
         for (int o = 0; o < 4; o++)
@@ -837,9 +1209,7 @@ __global__ __launch_bounds__(256) void k_contract(const WorkItem *__restrict__ i
                     if (tid < (la + 1) * (lb + 1)) {
                         const int nA = tid / (lb + 1), nB = tid % (lb + 1);
                         if (dev_lam_lo(L, nA) == 0 && dev_lam_lo(L, nB) == 0) {
-                            int ka = 0, kb = 0;
-                            for (int n = 0; n < nA; n++) ka += dev_lam_cnt(L, n);
-                            for (int n = 0; n < nB; n++) kb += dev_lam_cnt(L, n);
+                            const int ka = dev_colbase(L, nA), kb = dev_colbase(L, nB);
                             double v = 0.0;
                             for (int i = 0; i < sa.nprim; i++) {
                                 const double ci_ = J.craw[sa.poff + i] * gnorm(la, J.alpha[sa.poff + i]);
@@ -864,43 +1234,46 @@ __global__ __launch_bounds__(256) void k_contract(const WorkItem *__restrict__ i
                     }
                 }
                 __syncthreads();
-                const double *PA = J.P + J.slot_poff[(long long) p.slot * J.nshell + it.a];
+                const double *PA = J.P + (is_so ? J.slot_pb3 : J.slot_pb2)[(long long) p.slot * J.nshell + it.a];
                 const double *PB = J.P + J.slot_poff[(long long) p.slot * J.nshell + it.b];
                 const double *PLB = is_so ? J.P + J.slot_ploff[(long long) p.slot * J.nshell + it.b] : nullptr;
                 const double fac = FOURPI2 * p.scale;
-                for (int m = 0; m < nm; m++) {
-                    // T[i][k2] = sum_k1 P^A[i][m][k1] Q[k1][k2]
-                    for (int e = tid; e < (r1 - r0) * KB; e += blockDim.x) {
-                        const int i = r0 + e / KB, k2 = e % KB;
-                        const double *pa = PA + ((long long) i * nm + m) * KA;
-                        double s = 0.0;
-                        for (int k1 = 0; k1 < KA; k1++) s += pa[k1] * Qs[k1 * KB + k2];
-                        Ts[e] = s;
-                    }
-                    __syncthreads();
+                // T[m][i][k2] = sum_k1 P^A[i][m][k1] Q[k1][k2] for all m at once
+                for (int e = tid; e < nm * nr * KB; e += blockDim.x) {
+                    const int k2 = e % KB, mi = e / KB, i = mi % nr, m = mi / nr;
+                    const double *pa = PA + ((long long) (r0 + i) * nm + m) * KA;
+                    double sacc = 0.0;
+                    for (int k1 = 0; k1 < KA; k1++) sacc += pa[k1] * Qs[k1 * KB + k2];
+                    Ts[e] = sacc;
+                }
+                __syncthreads();
 #pragma unroll
-                    for (int q = 0; q < CT_EPT; q++) {
-                        const int e = tid + q * 256;
-                        if (e >= nel) break;
-                        const int i = e / ncB, j = e % ncB;
-                        const double *tr = Ts + i * KB;
-                        if (!is_so) {
+                for (int q = 0; q < CT_EPT; q++) {
+                    const int e = tid + q * 256;
+                    if (e >= nel) break;
+                    const int i = e / ncB, j = e % ncB;
+                    if (!is_so) {
+                        double sacc = 0.0;
+                        for (int m = 0; m < nm; m++) {
+                            const double *tr = Ts + (m * nr + i) * KB;
                             const double *pb = PB + ((long long) j * nm + m) * KB;
-                            double s = 0.0;
-                            for (int k2 = 0; k2 < KB; k2++) s += tr[k2] * pb[k2];
-                            acc[0][q] += fac * s;
-                        } else {
+                            for (int k2 = 0; k2 < KB; k2++) sacc += tr[k2] * pb[k2];
+                        }
+                        acc[0][q] += fac * sacc;
+                    } else {
 #pragma unroll
-                            for (int eta = 0; eta < 3; eta++) {
+                        for (int eta = 0; eta < 3; eta++) {
+                            double sacc = 0.0;
+                            for (int m = 0; m < nm; m++) {
+                                const double *tr = Ts + (m * nr + i) * KB;
                                 const double *pl = PLB + (((long long) eta * ncB + j) * nm + m) * KB;
-                                double s = 0.0;
-                                for (int k2 = 0; k2 < KB; k2++) s += tr[k2] * pl[k2];
-                                acc[1 + eta][q] += fac * s;
+                                for (int k2 = 0; k2 < KB; k2++) sacc += tr[k2] * pl[k2];
                             }
+                            acc[1 + eta][q] += fac * sacc;
                         }
                     }
-                    __syncthreads();
                 }
+                __syncthreads();
             }
             // ---- outercore projectors ----
             if ((parts & 8) && c.oe > c.ob) {
@@ -1085,7 +1458,7 @@ struct eng_plan {
     DevJob J{};
     std::vector<void *> bufs;
     double *d_out = nullptr, *d_F = nullptr, *d_wU = nullptr, *d_Q = nullptr, *d_P = nullptr, *d_OCV = nullptr,
-           *d_delta = nullptr;
+           *d_delta = nullptr, *d_scr = nullptr, *d_Qc = nullptr;
     TabItem *d_tab = nullptr;
     int n_tab = 0;
     int *d_wpots = nullptr;
@@ -1112,6 +1485,8 @@ struct eng_plan {
     double last_seconds = 0.0, last_radial_seconds = 0.0;
     int min_level = 0;
     double tol = 1e-16;
+    int radial_mfma = 0;
+    int nm_max = 1;
 };
 
 template <class T>
@@ -1303,6 +1678,7 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     std::vector<int> slot_rows(nslot), slot_M(nslot);
     std::vector<int> slot_rowoff((size_t) nslot * nshell, -1);
     std::vector<long long> slot_poff((size_t) nslot * nshell, -1), slot_ploff((size_t) nslot * nshell, -1);
+    std::vector<long long> slot_pb2((size_t) nslot * nshell, -1), slot_pb3((size_t) nslot * nshell, -1);
     std::vector<ProjItem> proj;
     long long plen = 0;
     for (int s = 0; s < nslot; s++) {
@@ -1317,6 +1693,8 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
                     RowDesc r;
                     r.frow = fb + lam_lo(L, n) + 2 * j;
                     r.n = n;
+                    r.shell = A;
+                    r.lam = lam_lo(L, n) + 2 * j;
                     rows.push_back(r);
                     M++;
                 }
@@ -1325,15 +1703,22 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
             pi.A = A;
             pi.L = L;
             pi.K = kcols_host(L, sh[A].L);
+            const long long psz = (long long) ncart(sh[A].L) * (2 * L + 1) * pi.K;
             pi.poff = plen;
-            plen += (long long) ncart(sh[A].L) * (2 * L + 1) * pi.K;
+            plen += psz;
+            pi.pb2 = plen;
+            plen += psz;
+            pi.pb3 = plen;
+            plen += psz;
             pi.ploff = -1;
             if (slots[s].has_so) {
                 pi.ploff = plen;
-                plen += 3LL * ncart(sh[A].L) * (2 * L + 1) * pi.K;
+                plen += 3 * psz;
             }
             slot_poff[(size_t) s * nshell + A] = pi.poff;
             slot_ploff[(size_t) s * nshell + A] = pi.ploff;
+            slot_pb2[(size_t) s * nshell + A] = pi.pb2;
+            slot_pb3[(size_t) s * nshell + A] = pi.pb3;
             proj.push_back(pi);
         }
         slot_M[s] = M;
@@ -1354,6 +1739,9 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
         q.maxlev = NLEV - 1;
         q.qoff = qlen;
         q.thr = p.thr;
+        q.center = slots[p.slot].c;
+        q.potential = k;
+        q.L = p.L;
         p.qoff = qlen;
         p.M = q.M;
         qlen += (long long) q.M * q.M;
@@ -1419,7 +1807,7 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     DPot *d_pot;
     int *d_tpow, *d_cidx, *d_slot_rowoff;
     OcEntry *d_oce;
-    long long *d_slot_poff, *d_slot_ploff, *d_ocv_tab;
+    long long *d_slot_poff, *d_slot_ploff, *d_ocv_tab, *d_slot_pb2, *d_slot_pb3;
     int rc = 0;
     rc |= dput(P, &d_sh, sh);
     rc |= dput(P, &d_alpha, alpha);
@@ -1435,6 +1823,8 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     rc |= dput(P, &d_slot_rowoff, slot_rowoff);
     rc |= dput(P, &d_slot_poff, slot_poff);
     rc |= dput(P, &d_slot_ploff, slot_ploff);
+    rc |= dput(P, &d_slot_pb2, slot_pb2);
+    rc |= dput(P, &d_slot_pb3, slot_pb3);
     rc |= dput(P, &d_ocv_tab, ocv_tab);
     rc |= dput(P, &P->d_tab, tab);
     rc |= dput(P, &P->d_wpots, wpots);
@@ -1448,6 +1838,8 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     rc |= dalloc(P, &P->d_F, (size_t) frows * NPTS);
     rc |= dalloc(P, &P->d_wU, (size_t) wpots.size() * NPTS);
     rc |= dalloc(P, &P->d_Q, (size_t) qlen);
+    rc |= dalloc(P, &P->d_scr, (size_t) qlen);
+    rc |= dalloc(P, &P->d_Qc, (size_t) qlen);
     rc |= dalloc(P, &P->d_P, (size_t) plen);
     rc |= dalloc(P, &P->d_OCV, (size_t) ocvlen);
     rc |= dalloc(P, &P->d_delta, (size_t) delta_len);
@@ -1464,6 +1856,7 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     P->n_ocv = (int) ocv.size();
     P->n_delta_centers = (int) delta_centers.size();
     P->n_items = nitem;
+    P->nm_max = 2 * Lmax_pot + 1;
 
     DevJob &J = P->J;
     J.sh = d_sh;
@@ -1480,14 +1873,21 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     J.slot_rowoff = d_slot_rowoff;
     J.slot_poff = d_slot_poff;
     J.slot_ploff = d_slot_ploff;
+    J.slot_pb2 = d_slot_pb2;
+    J.slot_pb3 = d_slot_pb3;
     J.ocv_tab = d_ocv_tab;
     J.delta = P->d_delta;
-    J.Q = P->d_Q;
+    J.Q = P->d_Qc;   // the contraction reads the canonicalised table
     J.P = P->d_P;
     J.OCV = P->d_OCV;
     J.nshell = nshell;   // lookup tables are indexed by basis shell; outercore shells follow them in sh
     J.op_stride = op_stride;
     for (int i = 0; i < 4; i++) HIPCHK(hipEventCreate(&P->ev[i]));
+    {
+        // radial quadrature on the matrix cores unless GRPP_RADIAL_KERNEL=valu
+        const char *k = getenv("GRPP_RADIAL_KERNEL");
+        P->radial_mfma = !(k && strcmp(k, "valu") == 0);
+    }
     *plan_out = P;
     return 0;
 }
@@ -1505,10 +1905,21 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
         dim3 g(NPTS / 256, P->n_wpots);
         hipLaunchKernelGGL(k_tab_wU, g, dim3(256), 0, st, P->d_wpots, P->J, P->T, P->d_wU);
     }
-    HIPCHK(hipEventRecord(P->ev[1], st));
+    if (P->n_tiles == 0) HIPCHK(hipEventRecord(P->ev[1], st));
+    if (P->n_tiles > 0) {
+        hipLaunchKernelGGL(k_screen, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->J, P->T,
+                           P->d_scr);
+        HIPCHK(hipEventRecord(P->ev[1], st));
+        if (P->radial_mfma)
+            hipLaunchKernelGGL(k_radial_mfma, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs, P->d_rows,
+                               P->d_F, P->T.rpow, P->d_wU, P->d_Q, P->min_level, P->tol, P->d_counters, P->d_scr, P->J.sh);
+        else
+            hipLaunchKernelGGL(k_radial, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->d_F,
+                               P->T.rpow, P->d_wU, P->d_Q, P->min_level, P->tol, P->d_counters, P->d_scr, P->J.sh);
+    }
     if (P->n_tiles > 0)
-        hipLaunchKernelGGL(k_radial, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->d_F,
-                           P->T.rpow, P->d_wU, P->d_Q, P->min_level, P->tol, P->d_counters);
+        hipLaunchKernelGGL(k_canon, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->J, P->d_Q,
+                           P->d_Qc);
     HIPCHK(hipEventRecord(P->ev[2], st));
     if (P->n_proj > 0) hipLaunchKernelGGL(k_proj, dim3(P->n_proj), dim3(256), 0, st, P->d_proj, P->J, P->T, P->d_P);
     if (P->n_delta_centers > 0)
@@ -1518,7 +1929,7 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
     if (P->n_items > 0) {
         hipLaunchKernelGGL(k_type1, dim3(P->n_items), dim3(256), 0, st, P->d_items, P->J, P->T, P->d_out, P->parts);
         hipLaunchKernelGGL(k_contract, dim3(P->n_items), dim3(256), 0, st, P->d_items, P->J, P->T, P->d_out,
-                           P->parts);
+                           P->parts, P->nm_max);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(P->ev[3], st));

@@ -20,6 +20,7 @@ from libgrpp_amd.io import System
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORC_SO = os.path.join(HERE, "liborc.so")
+ORC_NOFMA_SO = os.path.join(HERE, "liborc_nofma.so")
 REF_SO = os.path.join(HERE, "_ref", "libgrpp_ref.so")
 
 ORC_NAMES = dict(new_shell="orc_new_shell", delete_shell="orc_delete_shell", new_potential="orc_new_potential",
@@ -105,6 +106,17 @@ class CpuLib:
 
 
 _cache = {}
+
+
+def oracle_nofma() -> CpuLib:
+    """The oracle compiled without floating-point contraction: the same algorithm under a second,
+    equally valid rounding.  Its spread against ``oracle()`` measures how far the reference
+    formulation is determined by its own arithmetic on a given input."""
+    if "nofma" not in _cache:
+        if not os.path.exists(ORC_NOFMA_SO):
+            raise RuntimeError("oracle/liborc_nofma.so is not built (make -C oracle)")
+        _cache["nofma"] = CpuLib(ORC_NOFMA_SO, "oracle")
+    return _cache["nofma"]
 
 
 def oracle() -> CpuLib:
