@@ -60,12 +60,23 @@ def main():
     ap.add_argument("--workload", default="heavy_cluster")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 256 MiB device-to-device copy (3 times): the known-size kernel "
+                         "against which the FETCH_SIZE / WRITE_SIZE counters are scaled")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "single-GPU build: --gpus must be 1 (multi-GPU sharding is design-only)"}))
         sys.exit(2)
 
     import torch  # device plumbing only: synchronisation around the timed region
+    if args.copy_calibration:
+        x = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": x.numel() * 8}))
+        return
     from libgrpp_amd import configs, lib as G
 
     sysm = configs.WORKLOADS[args.workload]()

@@ -319,68 +319,80 @@ __device__ __forceinline__ double bessel_ratio(const DevTables &T, int l, double
     return bessel_k(T.bes, T.rfun, a, x) / bessel_k(T.bes, T.rfun, b, x);
 }
 
-__device__ double screen_fixed_point_eq(const DevTables &T, double r, int n, int l1, int l2, double p, double k1,
-                                        double k2)
+// x^n for n >= 0 by binary powering
+__device__ __forceinline__ double ipow(double x, int n)
 {
-    const double k1r = k1 * r, k2r = k2 * r;
-    const double K1 = bessel_ratio(T, l1, k1r);
-    const double K2 = bessel_ratio(T, l2, k2r);
-    double a = K1 * k1r + K2 * k2r + n;
-    if (l1 > 0) a = a - l1 - 1;
-    if (l2 > 0) a = a - l2 - 1;
-    return sqrt(a / (2.0 * p));
+    double r = 1.0;
+    for (; n > 0; n >>= 1) {
+        if (n & 1) r *= x;
+        x *= x;
+    }
+    return r;
+}
+
+// One primitive triple of the envelope (screening.c:272-378).  Loop invariants of the triple are
+// hoisted: 1/(2p), sqrt(pi/p), sqrt(p) and the integer shift of the fixed-point equation
+//   r = sqrt((k1 r K_{l1-1}/K_{l1}(k1 r) + k2 r K_{l2-1}/K_{l2}(k2 r) + n - [l1>0](l1+1) - [l2>0](l2+1)) / 2p),
+// iterated from r = (aA CA + aB CB)/p until |r - r_prev| <= 1e-2; a triple that has not settled
+// after four evaluations fails the whole estimate (the reference's five-step rule).  The estimate
+// only steers the quadrature (screened-out threshold, forced refinement), so few-ulp differences
+// of these algebraically equal forms are immaterial.
+__device__ __forceinline__ bool screen_triple(const DevTables &T, int n, int l1, int l2, double aA, double aB,
+                                              double eta, double CA, double CB, double k1, double k2, double &val)
+{
+    const double p = aA + aB + eta, invp = 1.0 / p, inv2p = 0.5 * invp;
+    const double nsh = (double) (n - (l1 > 0 ? l1 + 1 : 0) - (l2 > 0 ? l2 + 1 : 0));
+    double r0 = (aA * CA + aB * CB) * invp, r0p;
+    int steps = 0;
+    do {
+        if (++steps == 5) return false;
+        r0p = r0;
+        const double k1r = k1 * r0, k2r = k2 * r0;
+        const double a = bessel_ratio(T, l1, k1r) * k1r + bessel_ratio(T, l2, k2r) * k2r + nsh;
+        r0 = sqrt(a * inv2p);
+    } while (fabs(r0 - r0p) > 1e-2);
+    const double dA = r0 - CA, dB = r0 - CB;
+    val = sqrt(M_PI * invp) * ipow(r0, n) * bessel_k(T.bes, T.rfun, l1, k1 * r0) * bessel_k(T.bes, T.rfun, l2, k2 * r0) *
+          exp(-eta * r0 * r0 - aA * dA * dA - aB * dB * dB) * 0.5 * (1.0 + erf(sqrt(p) * r0));
+    return true;
 }
 
 __device__ double screen_entry(const DevJob &J, const DevTables &T, const DShell &A, const DShell &B, double CA,
                                double CB, const DPot &P, int l1, int l2, int N)
 {
-    const double CA2 = CA * CA, CB2 = CB * CB;
     double est = 0.0;
+#pragma unroll 1
     for (int i = 0; i < A.nprim; i++) {
         const double ci = J.cnorm[A.poff + i];
         if (ci == 0.0) continue;                     // primitives dropped by libgrpp_shell_shrink
         const double aA = J.alpha[A.poff + i], k1 = 2 * aA * CA;
+#pragma unroll 1
         for (int j = 0; j < B.nprim; j++) {
             const double cj = J.cnorm[B.poff + j];
             if (cj == 0.0) continue;
             const double aB = J.alpha[B.poff + j], k2 = 2 * aB * CB;
+            // K_l(0) = 0 for l >= 1: the whole (i, j) block vanishes
+            if ((l1 >= 1 && fabs(k1) <= 1e-14) || (l2 >= 1 && fabs(k2) <= 1e-14)) continue;
+            const bool s_s = (l1 == 0 && l2 == 0 && fabs(k1) <= 1e-14 && fabs(k2) <= 1e-14);
+            const double cc = ci * cj;
+#pragma unroll 1
             for (int t = 0; t < P.nterm; t++) {
                 const double eta = J.tzeta[P.toff + t];
                 const int n = N + J.tpow[P.toff + t];
-                double val = 0.0;
-                if ((l1 >= 1 && fabs(k1) <= 1e-14) || (l2 >= 1 && fabs(k2) <= 1e-14)) {
-                    val = 0.0;
-                } else {
-                    const double p = aA + aB + eta;
-                    if (l1 == 0 && l2 == 0 && fabs(k1) <= 1e-14 && fabs(k2) <= 1e-14) {
-                        val = exp(-aA * CA * CA - aB * CB * CB) * int_gauss_s(n, p);
-                    } else {
-                        double r0 = (aA * CA + aB * CB) / p, r0p;
-                        int steps = 0;
-                        bool fail = false;
-                        do {
-                            steps++;
-                            if (steps == 5) { fail = true; break; }
-                            r0p = r0;
-                            r0 = screen_fixed_point_eq(T, r0, n, l1, l2, p, k1, k2);
-                        } while (fabs(r0 - r0p) > 1e-2);
-                        if (fail) return __longlong_as_double(0x7ff8000000000000LL);
-                        val = sqrt(M_PI / p) * pow(r0, n) * bessel_k(T.bes, T.rfun, l1, k1 * r0) *
-                              bessel_k(T.bes, T.rfun, l2, k2 * r0) *
-                              exp(-eta * r0 * r0 - aA * (r0 - CA) * (r0 - CA) - aB * (r0 - CB) * (r0 - CB)) * 0.5 *
-                              (1 + erf(sqrt(p) * r0));
-                    }
+                double val;
+                if (s_s) {
+                    val = exp(-aA * CA * CA - aB * CB * CB) * int_gauss_s(n, aA + aB + eta);
+                } else if (!screen_triple(T, n, l1, l2, aA, aB, eta, CA, CB, k1, k2, val)) {
+                    return __longlong_as_double(0x7ff8000000000000LL);
                 }
-                est += ci * cj * J.tcoef[P.toff + t] * val;
+                est += cc * J.tcoef[P.toff + t] * val;
             }
         }
     }
-    (void) CA2;
-    (void) CB2;
     return est;
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_screen(const TileItem *__restrict__ tiles, const QJob *__restrict__ jobs,
+__global__ __launch_bounds__(256) void k_screen(const TileItem *__restrict__ tiles, const QJob *__restrict__ jobs,
                                                 const RowDesc *__restrict__ rows, DevJob J, DevTables T,
                                                 double *__restrict__ scr)
 {
@@ -389,6 +401,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     const RowDesc *R = rows + jb.rows;
     const DCenter c = J.ctr[jb.center];
     const DPot P = J.pot[jb.potential];
+#pragma unroll 1
     for (int e = threadIdx.x; e < RT * RT; e += blockDim.x) {
         const int row = t.ti * RT + e / RT, col = t.tj * RT + e % RT;
         if (row >= jb.M || col >= jb.M) continue;
@@ -1753,6 +1766,17 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
         P->n_entries += (long long) q.M * (q.M + 1) / 2;
         flops_pp += 2.0 * RT * RT * nt * (nt + 1) / 2;
         (void) flops_pp;
+    }
+    // largest tiles first (entries of a diagonal tile count half): the long blocks start in the
+    // first wave of the launch instead of forming its tail
+    {
+        auto tile_entries = [&](const TileItem &t) {
+            const int M = jobs[t.job].M;
+            const long long ri = std::min(RT, M - t.ti * RT), rj = std::min(RT, M - t.tj * RT);
+            return (t.ti == t.tj) ? ri * (ri + 1) / 2 : ri * rj;
+        };
+        std::stable_sort(tiles.begin(), tiles.end(),
+                         [&](const TileItem &a, const TileItem &b) { return tile_entries(a) > tile_entries(b); });
     }
     // outercore vectors and deltas
     std::vector<OcvItem> ocv;
