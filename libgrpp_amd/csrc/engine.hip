@@ -1021,15 +1021,17 @@ __device__ void mmd_accumulate(const MmdShell A, const MmdShell B, double Cx, do
     (void) n_s;
 }
 
-constexpr int MMD_SCRATCH = 12288;   // doubles of LDS for E / R tables (96 KiB)
+constexpr int MMD_SCRATCH = 12288;   // doubles of LDS for E / R tables (96 KiB), the upper bound
+constexpr int MMD_SCRATCH_BATCH = 4096;   // doubles for a batch of primitive-pair E tables
 constexpr int MAX_BLOCK = 66 * 66;
 constexpr int MAX_OCB = 66 * 28;   // basis shell x outercore shell (L <= 6)
 
 __global__ __launch_bounds__(256) void k_type1(const WorkItem *__restrict__ items, DevJob J, DevTables T,
-                                               double *__restrict__ out, int parts)
+                                               double *__restrict__ out, int parts, int scratch_len)
 {
-    __shared__ double scratch[MMD_SCRATCH];
-    __shared__ double acc[MAX_BLOCK];
+    // dynamic LDS sized by the plan: E / R tables (scratch_len doubles), then the block accumulator
+    extern __shared__ double t1_lds[];
+    double *scratch = t1_lds, *acc = t1_lds + scratch_len;
     const WorkItem it = items[blockIdx.x];
     const DShell sa = J.sh[it.a], sb = J.sh[it.b];
     const int ncA = dev_ncart(sa.L), ncB = dev_ncart(sb.L), nel = ncA * ncB;
@@ -1043,7 +1045,7 @@ __global__ __launch_bounds__(256) void k_type1(const WorkItem *__restrict__ item
             if (c.local < 0) continue;
             const DPot p = J.pot[c.local];
             mmd_accumulate(A, B, c.x, c.y, c.z, J.tpow, J.tcoef, J.tzeta, p.toff, p.toff + p.nterm, J.alpha, J.craw,
-                           acc, scratch, MMD_SCRATCH, T.boys);
+                           acc, scratch, scratch_len, T.boys);
         }
     }
     for (int e = threadIdx.x; e < nel; e += blockDim.x) {
@@ -1161,7 +1163,7 @@ __global__ void k_delta(const int *__restrict__ centers, int n, DevJob J, double
 }
 
 // ================================================================== contraction
-constexpr int CT_EPT = 8;   // output elements per thread (<= 2048 per row chunk)
+constexpr int CT_EPT = 1;   // output elements per thread (<= 256 per row chunk)
 
 __device__ __forceinline__ double int_gauss(int n, double a)
 {
@@ -1177,20 +1179,22 @@ __device__ __forceinline__ double int_gauss(int n, double a)
     return fk / (2.0 * pow(a, k + 1));
 }
 
-constexpr int TS_MAX = 6144;   // doubles of LDS for the (m, i, k2) intermediate
+constexpr int TS_MAX = 6144;   // upper bound of the LDS doubles for the (m, i, k2) intermediate
 
 __global__ __launch_bounds__(256) void k_contract(const WorkItem *__restrict__ items, DevJob J, DevTables T,
-                                                  double *__restrict__ out, int parts, int nm_max)
+                                                  double *__restrict__ out, int parts, int nm_max, int qs_len,
+                                                  int ts_len)
 {
-    __shared__ double Qs[66 * 66];
-    __shared__ double Ts[TS_MAX];
+    // dynamic LDS sized by the plan: Q block (qs_len doubles) + (m, i, k2) intermediate (ts_len)
+    extern __shared__ double ct_lds[];
+    double *Qs = ct_lds, *Ts = ct_lds + qs_len;
     const WorkItem it = items[blockIdx.x];
     const DShell sa = J.sh[it.a], sb = J.sh[it.b];
     const int la = sa.L, lb = sb.L, ncA = dev_ncart(la), ncB = dev_ncart(lb);
     const int tid = threadIdx.x;
     const double FOURPI2 = 16.0 * M_PI * M_PI;
     // rows of A per pass: the outputs fit CT_EPT per thread and T[m][i][k2] fits TS_MAX
-    const int rows_per_chunk = max(1, min((CT_EPT * 256) / ncB, TS_MAX / (nm_max * ncB)));
+    const int rows_per_chunk = max(1, min((CT_EPT * 256) / ncB, ts_len / (nm_max * ncB)));
     for (int r0 = 0; r0 < ncA; r0 += rows_per_chunk) {
         const int r1 = min(ncA, r0 + rows_per_chunk), nr = r1 - r0, nel = nr * ncB;
         double acc[4][CT_EPT];
@@ -1500,6 +1504,8 @@ struct eng_plan {
     double tol = 1e-16;
     int radial_mfma = 0;
     int nm_max = 1;
+    int ct_qs = 1, ct_ts = 1;   // k_contract LDS doubles
+    int t1_scratch = 64, t1_acc = 1;   // k_type1 LDS doubles
 };
 
 template <class T>
@@ -1819,6 +1825,25 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
         P->n_integrals += 4LL * ncart(sh[e.a].L) * ncart(sh[e.b].L) * ntask;
     }
     std::vector<int> cidx(center_index, center_index + nidx);
+    {
+        // LDS of k_type1: E tables of a batch of primitive pairs (r^0 terms), or E plus two R/G
+        // layers of one pair (r^-1, r^-2 terms); the block accumulator
+        bool other_pow = false;
+        for (int v : tpow) other_pow |= (v != 2);
+        long long sc = 64, acc = 1;
+        for (int i = 0; i < nitem; i++) {
+            const int la = sh[items[i].a].L, lb = sh[items[i].b].L, N = la + lb;
+            const long long esz = 3LL * (la + 1) * (lb + 1) * (N + 1);
+            const long long ssz = (long long) (N + 1) * (N + 2) * (N + 3) / 6;
+            const long long npair = (long long) sh[items[i].a].nprim * sh[items[i].b].nprim;
+            sc = std::max(sc, std::min<long long>(MMD_SCRATCH_BATCH, npair * esz));
+            sc = std::max(sc, esz);
+            if (other_pow) sc = std::max(sc, esz + 2 * ssz + 64);
+            acc = std::max<long long>(acc, (long long) ncart(la) * ncart(lb));
+        }
+        P->t1_scratch = (int) std::min<long long>(sc, MMD_SCRATCH);
+        P->t1_acc = (int) acc;
+    }
 
     // ---------------- upload ----------------
     if (device_tables(Lmax_pot, lmax_shell, &P->T)) {
@@ -1881,6 +1906,16 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     P->n_delta_centers = (int) delta_centers.size();
     P->n_items = nitem;
     P->nm_max = 2 * Lmax_pot + 1;
+    {
+        // LDS of k_contract: the largest (lambda, n) block of one shell and the (m, i, k2) intermediate
+        int kmax = 1, ncmax = 1;
+        for (int A = 0; A < nshell; A++) {
+            ncmax = std::max(ncmax, ncart(sh[A].L));
+            for (int L = 0; L <= Lmax_pot; L++) kmax = std::max(kmax, kcols_host(L, sh[A].L));
+        }
+        P->ct_qs = kmax * kmax;
+        P->ct_ts = std::min(TS_MAX, P->nm_max * ncmax * kmax);
+    }
 
     DevJob &J = P->J;
     J.sh = d_sh;
@@ -1951,9 +1986,15 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
                            P->n_delta_centers, P->J, P->d_delta);
     if (P->n_ocv > 0) hipLaunchKernelGGL(k_ocv, dim3(P->n_ocv), dim3(256), 0, st, P->d_ocvi, P->J, P->T, P->d_OCV);
     if (P->n_items > 0) {
-        hipLaunchKernelGGL(k_type1, dim3(P->n_items), dim3(256), 0, st, P->d_items, P->J, P->T, P->d_out, P->parts);
-        hipLaunchKernelGGL(k_contract, dim3(P->n_items), dim3(256), 0, st, P->d_items, P->J, P->T, P->d_out,
-                           P->parts, P->nm_max);
+        // dynamic LDS beyond the 64 KiB default (up to the 160 KiB of a gfx950 CU) must be opted into
+        HIPCHK(hipFuncSetAttribute((const void *) k_type1, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (P->t1_scratch + P->t1_acc) * (int) sizeof(double)));
+        HIPCHK(hipFuncSetAttribute((const void *) k_contract, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (P->ct_qs + P->ct_ts) * (int) sizeof(double)));
+        hipLaunchKernelGGL(k_type1, dim3(P->n_items), dim3(256), (size_t) (P->t1_scratch + P->t1_acc) * sizeof(double),
+                           st, P->d_items, P->J, P->T, P->d_out, P->parts, P->t1_scratch);
+        hipLaunchKernelGGL(k_contract, dim3(P->n_items), dim3(256), (size_t) (P->ct_qs + P->ct_ts) * sizeof(double),
+                           st, P->d_items, P->J, P->T, P->d_out, P->parts, P->nm_max, P->ct_qs, P->ct_ts);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(P->ev[3], st));
