@@ -8,6 +8,7 @@ namespace grpp {
 
 struct DevTables {
     const double *r, *w, *rpow, *bes, *rfun, *boys;
+    const double *besh;       // Taylor coefficients over k! (Horner form), steering-only evaluations
     const double *rsh;
     const int *rsh_off, *comb, *comb_off;
     const double *lmat;

@@ -309,12 +309,17 @@ __device__ __forceinline__ double bessel_ratio(const DevTables &T, int l, double
 {
     const int a = (l == 0) ? 1 : l - 1, b = (l == 0) ? 0 : l;
     if (x > 1e-7 && x <= 16.0) {
+        // both orders from one node of the Horner-form table; the quotient by one reciprocal
+        // refinement (relative error ~1e-16, ample for the fixed-point iteration it steers)
         const int i0 = (int) (rint(x * 100) + 1e-5);
-        const double dx = x - i0 * 0.01, dx2 = dx * dx, dx3 = dx2 * dx, dx4 = dx3 * dx;
-        const double *da = T.bes + ((size_t) i0 * BES_NL + a) * 5, *db = T.bes + ((size_t) i0 * BES_NL + b) * 5;
-        const double ka = da[0] + da[1] * dx + da[2] * dx2 * 0.5 + da[3] * dx3 * (1.0 / 6.0) + da[4] * dx4 * (1.0 / 24.0);
-        const double kb = db[0] + db[1] * dx + db[2] * dx2 * 0.5 + db[3] * dx3 * (1.0 / 6.0) + db[4] * dx4 * (1.0 / 24.0);
-        return ka / kb;
+        const double dx = x - i0 * 0.01;
+        const double *da = T.besh + ((size_t) i0 * BES_NL + a) * 5, *db = T.besh + ((size_t) i0 * BES_NL + b) * 5;
+        const double ka = da[0] + dx * (da[1] + dx * (da[2] + dx * (da[3] + dx * da[4])));
+        const double kb = db[0] + dx * (db[1] + dx * (db[2] + dx * (db[3] + dx * db[4])));
+        double r = __builtin_amdgcn_rcp(kb);
+        r = fma(fma(-kb, r, 1.0), r, r);
+        r = fma(fma(-kb, r, 1.0), r, r);
+        return ka * r;
     }
     return bessel_k(T.bes, T.rfun, a, x) / bessel_k(T.bes, T.rfun, b, x);
 }
@@ -392,29 +397,52 @@ __device__ double screen_entry(const DevJob &J, const DevTables &T, const DShell
     return est;
 }
 
-__global__ __launch_bounds__(256) void k_screen(const TileItem *__restrict__ tiles, const QJob *__restrict__ jobs,
-                                                const RowDesc *__restrict__ rows, DevJob J, DevTables T,
-                                                double *__restrict__ scr)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_screen(
+    const TileItem *__restrict__ tiles, const QJob *__restrict__ jobs, const RowDesc *__restrict__ rows, DevJob J,
+    DevTables T, double *__restrict__ scr)
 {
-    const TileItem t = tiles[blockIdx.x];
+    // one wave per quarter tile (rows sub, sub+4, ...).  The wave first lists the entries it must
+    // estimate -- inside the table, upper triangle, canonical (lambda1, lambda2, N) split -- with the
+    // full-cost ones first and the ones that vanish analytically (K_l(0) = 0 for l >= 1 on a shell
+    // sitting on the centre) last, so lanes run entries of like cost side by side.
+    __shared__ int list[RT * RT / 4];
+    __shared__ int n_full, n_zero;
+    const TileItem t = tiles[blockIdx.x >> 2];
     const QJob jb = jobs[t.job];
     const RowDesc *R = rows + jb.rows;
     const DCenter c = J.ctr[jb.center];
     const DPot P = J.pot[jb.potential];
-#pragma unroll 1
-    for (int e = threadIdx.x; e < RT * RT; e += blockDim.x) {
-        const int row = t.ti * RT + e / RT, col = t.tj * RT + e % RT;
+    const int sub = blockIdx.x & 3;
+    if (threadIdx.x == 0) {
+        n_full = 0;
+        n_zero = 0;
+    }
+    __syncthreads();
+    constexpr int NE = RT * RT / 4;
+    for (int e = threadIdx.x; e < NE; e += blockDim.x) {
+        const int row = t.ti * RT + (e / RT) * 4 + sub, col = t.tj * RT + e % RT;
         if (row >= jb.M || col >= jb.M) continue;
         if (t.ti == t.tj && row > col) continue;
+        const RowDesc ra = R[row], rb = R[col];
+        const DShell A = J.sh[ra.shell], B = J.sh[rb.shell];
+        const int N = ra.n + rb.n;
+        const bool a_first = (ra.shell < rb.shell) || (ra.shell == rb.shell && ra.lam <= rb.lam);
+        const int cn = a_first ? dev_canon_nA(jb.L, A.L, B.L, ra.lam, rb.lam, N)
+                               : dev_canon_nA_max(jb.L, A.L, B.L, ra.lam, rb.lam, N);
+        if (cn != ra.n) continue;
+        const bool a_on = (c.x - A.x) * (c.x - A.x) + (c.y - A.y) * (c.y - A.y) + (c.z - A.z) * (c.z - A.z) == 0.0;
+        const bool b_on = (c.x - B.x) * (c.x - B.x) + (c.y - B.y) * (c.y - B.y) + (c.z - B.z) * (c.z - B.z) == 0.0;
+        const bool zero = (a_on && ra.lam >= 1) || (b_on && rb.lam >= 1);
+        if (zero) list[NE - 1 - atomicAdd(&n_zero, 1)] = e;
+        else list[atomicAdd(&n_full, 1)] = e;
+    }
+    __syncthreads();
+    const int nf = n_full, nz = n_zero;
+#pragma unroll 1
+    for (int q = threadIdx.x; q < nf + nz; q += blockDim.x) {
+        const int e = (q < nf) ? list[q] : list[NE - nz + (q - nf)];
+        const int row = t.ti * RT + (e / RT) * 4 + sub, col = t.tj * RT + e % RT;
         RowDesc ra = R[row], rb = R[col];
-        {
-            // only the canonical split of (lambda1, lambda2, N) is screened; the others read it
-            const int la = J.sh[ra.shell].L, lb = J.sh[rb.shell].L, N = ra.n + rb.n;
-            const bool a_first = (ra.shell < rb.shell) || (ra.shell == rb.shell && ra.lam <= rb.lam);
-            const int cn = a_first ? dev_canon_nA(jb.L, la, lb, ra.lam, rb.lam, N)
-                                   : dev_canon_nA_max(jb.L, la, lb, ra.lam, rb.lam, N);
-            if (cn != ra.n) continue;
-        }
         // a fixed orientation of the (A, B, lambda1, lambda2, N) tuple so that every split and both
         // triangles get bitwise the same estimate
         if (ra.shell > rb.shell || (ra.shell == rb.shell && ra.lam > rb.lam)) {
@@ -1409,6 +1437,8 @@ static int device_tables(int Lmax, int nmax, DevTables *out)
         D.t.rpow = p;
         if (upload(H.bes, &p, D.bufs)) return 1;
         D.t.bes = p;
+        if (upload(H.besh, &p, D.bufs)) return 1;
+        D.t.besh = p;
         if (upload(H.rfun, &p, D.bufs)) return 1;
         D.t.rfun = p;
         if (upload(H.boys, &p, D.bufs)) return 1;
@@ -1966,7 +1996,7 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
     }
     if (P->n_tiles == 0) HIPCHK(hipEventRecord(P->ev[1], st));
     if (P->n_tiles > 0) {
-        hipLaunchKernelGGL(k_screen, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->J, P->T,
+        hipLaunchKernelGGL(k_screen, dim3(4 * P->n_tiles), dim3(64), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->J, P->T,
                            P->d_scr);
         HIPCHK(hipEventRecord(P->ev[1], st));
         if (P->radial_mfma)

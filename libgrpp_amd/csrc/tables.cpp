@@ -157,6 +157,11 @@ static void build(HostTables &T)
     for (int i = 0; i < BES_NPT; i++)
         for (int l = 0; l < BES_NL; l++)
             for (int k = 0; k < 5; k++) T.bes[((size_t) i * BES_NL + l) * 5 + k] = F(i, l, k);
+    T.besh.assign(T.bes.size(), 0.0);
+    {
+        const double inv_fact[5] = {1.0, 1.0, 0.5, 1.0 / 6.0, 1.0 / 24.0};
+        for (size_t e = 0; e < T.bes.size(); e++) T.besh[e] = T.bes[e] * inv_fact[e % 5];
+    }
     T.rfun.assign(BES_NL * BES_NL, 0.0);
     for (int n = 0; n < BES_NL; n++)
         for (int k = 0; k <= n; k++) {
