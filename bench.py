@@ -26,6 +26,20 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix), AMD published
 
 
+def traffic_bytes(kernel: str):
+    """HBM bytes per launch of ``kernel`` on this workload, from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads for a plain copy of
+    known size); None when no summary is present."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_heavy_cluster.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 2 and f[0] == kernel + ":" and f[1].isdigit():
+            return int(f[1])
+    return None
+
+
 def cpu_baseline(sysm, budget_s: float = 15.0):
     """Genuine libgrpp (compiled from its own sources into oracle/_ref) on a bounded sample of the
     same workload: every k-th ordered shell pair with all centres, OpenMP dynamic over pairs on
@@ -84,24 +98,29 @@ def main():
     for _ in range(args.warmup):
         plan.execute()
     torch.cuda.synchronize()
-    dev_times, rad_times = [], []
+    phases = ["tables_seconds", "screen_seconds", "radial_seconds", "canon_seconds", "proj_seconds",
+              "type1_seconds", "contract_seconds", "device_seconds"]
+    times = {k: [] for k in phases}
     t0 = time.perf_counter()
     for _ in range(args.steps):
         plan.execute()
-        st = plan.stats()
-        dev_times.append(st["device_seconds"])
-        rad_times.append(st["radial_seconds"])
+        st = plan.stats()   # per-phase HIP-event times of this step, on the plan's stream
+        for k in phases:
+            times[k].append(st[k])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     st = plan.stats()
     plan.close()
+    mean = {k: float(np.mean(v)) for k, v in times.items()}
 
     ntask = configs.n_tasks(sysm)
     nint = configs.n_integrals(sysm)
     ms = 1e3 * wall / args.steps
     tasks_per_s = ntask / (wall / args.steps)
-    rad = float(np.mean(rad_times))
-    achieved = st["fp64_flops"] / rad / 1e12
+    scr = mean["screen_seconds"]
+    achieved = st["screen_flops"] / scr / 1e12
+    rad = mean["radial_seconds"]
+    rad_achieved = st["fp64_flops"] / rad / 1e12
     line = {
         "metric": "FP64 GRPP shell-pair x centre tasks/s (full GRPP matrix build, plain adaptive radial grid)",
         "value": tasks_per_s,
@@ -120,11 +139,18 @@ def main():
                    "shells": len(sysm.shells), "centres": len(sysm.centers), "dim": sysm.dim, "tasks": ntask,
                    "integrals_per_step": nint, "seed": 0},
         "integrals_per_s": nint / (wall / args.steps),
-        "device_ms_per_step": 1e3 * float(np.mean(dev_times)),
-        "roofline": {"bound": "mfma", "kernel": "k_radial", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "radial_ms": rad * 1e3, "radial_flops": st["fp64_flops"], "radial_points": st["n_radial_points"],
-                     "radial_tiles": st["n_tiles"]},
+        "device_ms_per_step": 1e3 * mean["device_seconds"],
+        "kernel_ms": {k.replace("_seconds", ""): 1e3 * v for k, v in mean.items() if k != "device_seconds"},
+        # dominant kernel: the Shaw-Hill envelope estimates (FP64 VALU; FLOP equivalents per triple and
+        # per fixed-point evaluation as counted by the engine)
+        "roofline": {"bound": "mfma", "kernel": "k_screen", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_bytes("k_screen"),
+                     "screen_flops": st["screen_flops"], "screen_triples": st["screen_triples"],
+                     "screen_steps": st["screen_steps"], "screen_ms": 1e3 * scr,
+                     "radial_product": {"kernel": "k_radial_mfma", "achieved": rad_achieved,
+                                        "frac": rad_achieved / FP64_PEAK_TFLOPS, "ms": 1e3 * rad,
+                                        "flops": st["fp64_flops"], "points": st["n_radial_points"],
+                                        "tiles": st["n_tiles"], "traffic": traffic_bytes("k_radial_mfma")}},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(sysm, args.cpu_budget)

@@ -37,6 +37,16 @@ typedef struct {
     double fp64_flops;            /* algorithmic FP64 operations of the last execution */
     double device_seconds;        /* kernel time of the last execution (HIP events) */
     double radial_seconds;        /* time of the radial-quadrature kernel alone */
+    /* per-phase kernel times of the last execution (HIP events on the plan's stream) */
+    double tables_seconds;        /* grid tabulation of the shell factors and potentials */
+    double screen_seconds;        /* Shaw-Hill envelope estimates of the radial entries */
+    double canon_seconds;         /* canonical-split gather of the radial tables */
+    double proj_seconds;          /* angular projectors and outercore vectors */
+    double type1_seconds;         /* type-1 (local) McMurchie-Davidson blocks */
+    double contract_seconds;      /* angular x radial contraction and outercore terms */
+    long long screen_triples;     /* primitive triples of the envelope estimates */
+    long long screen_steps;       /* fixed-point evaluations of the envelope maxima */
+    double screen_flops;          /* envelope work in FLOP equivalents (see the engine) */
 } libgrpp_batch_stats_t;
 
 /*

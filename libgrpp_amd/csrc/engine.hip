@@ -304,22 +304,57 @@ __device__ __forceinline__ long long canon_index(const RowDesc *R, const DShell 
 // the integrand maximum r0 is found by <= 4 fixed-point steps of tolerance 1e-2 using Bessel ratios;
 // any triple that does not settle makes the screening inert.  Inert screening is encoded as NaN,
 // which the quadrature's tests treat exactly as the reference treats a failed screening.
-// K_{l-1}(x)/K_l(x) (l > 0) or K_1(x)/K_0(x) (l = 0): both orders from one table node
+//
+// Steering-only evaluations of K_l for the envelope estimates: the reference's scheme (table
+// Taylor quartic for x <= 16, asymptotic R_l(-x)/(2x) beyond) in Horner form with one reciprocal.
+__device__ __forceinline__ double rcp_refined(double b)
+{
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(fma(-b, r, 1.0), r, r);
+    return fma(fma(-b, r, 1.0), r, r);
+}
+
+__device__ __forceinline__ double bessel_k_steer(const DevTables &T, int n, double x)
+{
+    if (x > 1e-7 && x <= 16.0) {
+        const int i0 = (int) (rint(x * 100) + 1e-5);
+        const double dx = x - i0 * 0.01;
+        const double *d = T.besh + ((size_t) i0 * BES_NL + n) * 5;
+        return d[0] + dx * (d[1] + dx * (d[2] + dx * (d[3] + dx * d[4])));
+    }
+    if (x > 16.0) {
+        const double inv = -rcp_refined(x);
+        const double *c = T.rfun + n * BES_NL;
+        double s = c[n];
+        for (int k = n - 1; k >= 0; k--) s = fma(s, inv, c[k]);
+        return s * (-0.5 * inv);
+    }
+    return bessel_k(T.bes, T.rfun, n, x);
+}
+
+// K_{l-1}(x)/K_l(x) (l > 0) or K_1(x)/K_0(x) (l = 0): both orders from one table node, or both
+// asymptotic polynomials in one pass (their common 1/(2x) cancels)
 __device__ __forceinline__ double bessel_ratio(const DevTables &T, int l, double x)
 {
     const int a = (l == 0) ? 1 : l - 1, b = (l == 0) ? 0 : l;
     if (x > 1e-7 && x <= 16.0) {
-        // both orders from one node of the Horner-form table; the quotient by one reciprocal
-        // refinement (relative error ~1e-16, ample for the fixed-point iteration it steers)
         const int i0 = (int) (rint(x * 100) + 1e-5);
         const double dx = x - i0 * 0.01;
         const double *da = T.besh + ((size_t) i0 * BES_NL + a) * 5, *db = T.besh + ((size_t) i0 * BES_NL + b) * 5;
         const double ka = da[0] + dx * (da[1] + dx * (da[2] + dx * (da[3] + dx * da[4])));
         const double kb = db[0] + dx * (db[1] + dx * (db[2] + dx * (db[3] + dx * db[4])));
-        double r = __builtin_amdgcn_rcp(kb);
-        r = fma(fma(-kb, r, 1.0), r, r);
-        r = fma(fma(-kb, r, 1.0), r, r);
-        return ka * r;
+        return ka * rcp_refined(kb);
+    }
+    if (x > 16.0) {
+        const double inv = -rcp_refined(x);
+        const double *ca = T.rfun + a * BES_NL, *cb = T.rfun + b * BES_NL;
+        const int nmax = a > b ? a : b;
+        double sa = 0.0, sb = 0.0;
+        for (int k = nmax; k >= 0; k--) {
+            sa = fma(sa, inv, k <= a ? ca[k] : 0.0);
+            sb = fma(sb, inv, k <= b ? cb[k] : 0.0);
+        }
+        return sa * rcp_refined(sb);
     }
     return bessel_k(T.bes, T.rfun, a, x) / bessel_k(T.bes, T.rfun, b, x);
 }
@@ -343,7 +378,8 @@ __device__ __forceinline__ double ipow(double x, int n)
 // only steers the quadrature (screened-out threshold, forced refinement), so few-ulp differences
 // of these algebraically equal forms are immaterial.
 __device__ __forceinline__ bool screen_triple(const DevTables &T, int n, int l1, int l2, double aA, double aB,
-                                              double eta, double CA, double CB, double k1, double k2, double &val)
+                                              double eta, double CA, double CB, double k1, double k2, double &val,
+                                              unsigned &nstep)
 {
     const double p = aA + aB + eta, invp = 1.0 / p, inv2p = 0.5 * invp;
     const double nsh = (double) (n - (l1 > 0 ? l1 + 1 : 0) - (l2 > 0 ? l2 + 1 : 0));
@@ -351,55 +387,61 @@ __device__ __forceinline__ bool screen_triple(const DevTables &T, int n, int l1,
     int steps = 0;
     do {
         if (++steps == 5) return false;
+        nstep++;
         r0p = r0;
         const double k1r = k1 * r0, k2r = k2 * r0;
         const double a = bessel_ratio(T, l1, k1r) * k1r + bessel_ratio(T, l2, k2r) * k2r + nsh;
         r0 = sqrt(a * inv2p);
     } while (fabs(r0 - r0p) > 1e-2);
     const double dA = r0 - CA, dB = r0 - CB;
-    val = sqrt(M_PI * invp) * ipow(r0, n) * bessel_k(T.bes, T.rfun, l1, k1 * r0) * bessel_k(T.bes, T.rfun, l2, k2 * r0) *
+    val = sqrt(M_PI * invp) * ipow(r0, n) * bessel_k_steer(T, l1, k1 * r0) * bessel_k_steer(T, l2, k2 * r0) *
           exp(-eta * r0 * r0 - aA * dA * dA - aB * dB * dB) * 0.5 * (1.0 + erf(sqrt(p) * r0));
     return true;
 }
 
-__device__ double screen_entry(const DevJob &J, const DevTables &T, const DShell &A, const DShell &B, double CA,
-                               double CB, const DPot &P, int l1, int l2, int N)
+// The envelope of one radial entry (screening.c:193-248), evaluated by a whole wave: lane q takes
+// the primitive triples q, q+64, ... of the (i, j, t) space, so the lanes of a wave share the entry's
+// (lambda1, lambda2, N) and run the same code.  A triple that fails the fixed point fails the
+// entry (NaN, the reference's failed screening); the lane sums are combined by an xor butterfly,
+// which leaves the bitwise same total in every lane.
+__device__ double screen_entry_wave(const DevJob &J, const DevTables &T, const DShell &A, const DShell &B,
+                                    double CA, double CB, const DPot &P, int l1, int l2, int N, unsigned &ntrip,
+                                    unsigned &nstep)
 {
+    const int lane = threadIdx.x & 63;
+    const int nt = P.nterm, njt = B.nprim * nt, ntot = A.nprim * njt;
     double est = 0.0;
 #pragma unroll 1
-    for (int i = 0; i < A.nprim; i++) {
-        const double ci = J.cnorm[A.poff + i];
-        if (ci == 0.0) continue;                     // primitives dropped by libgrpp_shell_shrink
-        const double aA = J.alpha[A.poff + i], k1 = 2 * aA * CA;
-#pragma unroll 1
-        for (int j = 0; j < B.nprim; j++) {
-            const double cj = J.cnorm[B.poff + j];
-            if (cj == 0.0) continue;
+    for (int q0 = 0; q0 < ntot; q0 += 64) {
+        const int q = q0 + lane;
+        bool fail = false;
+        if (q < ntot) {
+            const int i = q / njt, r = q - i * njt, j = r / nt, t = r - j * nt;
+            const double ci = J.cnorm[A.poff + i], cj = J.cnorm[B.poff + j];
+            const double aA = J.alpha[A.poff + i], k1 = 2 * aA * CA;
             const double aB = J.alpha[B.poff + j], k2 = 2 * aB * CB;
-            // K_l(0) = 0 for l >= 1: the whole (i, j) block vanishes
-            if ((l1 >= 1 && fabs(k1) <= 1e-14) || (l2 >= 1 && fabs(k2) <= 1e-14)) continue;
-            const bool s_s = (l1 == 0 && l2 == 0 && fabs(k1) <= 1e-14 && fabs(k2) <= 1e-14);
-            const double cc = ci * cj;
-#pragma unroll 1
-            for (int t = 0; t < P.nterm; t++) {
+            // primitives dropped by libgrpp_shell_shrink; K_l(0) = 0 for l >= 1
+            if (ci != 0.0 && cj != 0.0 && !((l1 >= 1 && fabs(k1) <= 1e-14) || (l2 >= 1 && fabs(k2) <= 1e-14))) {
                 const double eta = J.tzeta[P.toff + t];
                 const int n = N + J.tpow[P.toff + t];
-                double val;
-                if (s_s) {
+                double val = 0.0;
+                ntrip++;
+                if (l1 == 0 && l2 == 0 && fabs(k1) <= 1e-14 && fabs(k2) <= 1e-14)
                     val = exp(-aA * CA * CA - aB * CB * CB) * int_gauss_s(n, aA + aB + eta);
-                } else if (!screen_triple(T, n, l1, l2, aA, aB, eta, CA, CB, k1, k2, val)) {
-                    return __longlong_as_double(0x7ff8000000000000LL);
-                }
-                est += cc * J.tcoef[P.toff + t] * val;
+                else
+                    fail = !screen_triple(T, n, l1, l2, aA, aB, eta, CA, CB, k1, k2, val, nstep);
+                if (!fail) est += ci * cj * J.tcoef[P.toff + t] * val;
             }
         }
+        if (__any(fail)) return __longlong_as_double(0x7ff8000000000000LL);
     }
+    for (int o = 32; o > 0; o >>= 1) est += __shfl_xor(est, o);
     return est;
 }
 
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_screen(
     const TileItem *__restrict__ tiles, const QJob *__restrict__ jobs, const RowDesc *__restrict__ rows, DevJob J,
-    DevTables T, double *__restrict__ scr)
+    DevTables T, double *__restrict__ scr, unsigned long long *__restrict__ counters)
 {
     // one wave per quarter tile (rows sub, sub+4, ...).  The wave first lists the entries it must
     // estimate -- inside the table, upper triangle, canonical (lambda1, lambda2, N) split -- with the
@@ -438,9 +480,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     }
     __syncthreads();
     const int nf = n_full, nz = n_zero;
+    unsigned ntrip = 0, nstep = 0;   // primitive triples and fixed-point evaluations (work accounting)
+    // analytically vanishing entries: every triple is zero, so the estimate is 0 (screened out)
+    for (int q = threadIdx.x; q < nz; q += blockDim.x) {
+        const int e = list[NE - nz + q];
+        const int row = t.ti * RT + (e / RT) * 4 + sub, col = t.tj * RT + e % RT;
+        scr[jb.qoff + (long long) row * jb.M + col] = 0.0;
+        scr[jb.qoff + (long long) col * jb.M + row] = 0.0;
+    }
+    // the others: the whole wave per entry
 #pragma unroll 1
-    for (int q = threadIdx.x; q < nf + nz; q += blockDim.x) {
-        const int e = (q < nf) ? list[q] : list[NE - nz + (q - nf)];
+    for (int q = 0; q < nf; q++) {
+        const int e = list[q];
         const int row = t.ti * RT + (e / RT) * 4 + sub, col = t.tj * RT + e % RT;
         RowDesc ra = R[row], rb = R[col];
         // a fixed orientation of the (A, B, lambda1, lambda2, N) tuple so that every split and both
@@ -453,9 +504,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
         const DShell A = J.sh[ra.shell], B = J.sh[rb.shell];
         const double ax = c.x - A.x, ay = c.y - A.y, az = c.z - A.z, bx = c.x - B.x, by = c.y - B.y, bz = c.z - B.z;
         const double CA = sqrt(ax * ax + ay * ay + az * az), CB = sqrt(bx * bx + by * by + bz * bz);
-        const double v = screen_entry(J, T, A, B, CA, CB, P, ra.lam, rb.lam, ra.n + rb.n);
-        scr[jb.qoff + (long long) row * jb.M + col] = v;
-        scr[jb.qoff + (long long) col * jb.M + row] = v;
+        const double v = screen_entry_wave(J, T, A, B, CA, CB, P, ra.lam, rb.lam, ra.n + rb.n, ntrip, nstep);
+        if (threadIdx.x == 0) {
+            scr[jb.qoff + (long long) row * jb.M + col] = v;
+            scr[jb.qoff + (long long) col * jb.M + row] = v;
+        }
+    }
+    if (counters) {
+        unsigned long long a = ntrip, b = nstep;
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o);
+            b += __shfl_xor(b, o);
+        }
+        if (threadIdx.x == 0) {
+            atomicAdd(&counters[3], a);
+            atomicAdd(&counters[4], b);
+        }
     }
 }
 
@@ -1523,13 +1587,15 @@ struct eng_plan {
     WorkItem *d_items = nullptr;
     int n_items = 0;
     unsigned long long *d_counters = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[8] = {};   // execution phases: see eng_plan_execute
     // statistics
     long long n_tasks = 0, n_integrals = 0, n_entries = 0;
     double flops_fixed = 0.0;    // contraction + type-1 + projector work (grid independent)
     double flops_per_point = 0.0; // radial FLOPs per grid point summed over tiles (x points below)
     unsigned long long last_points = 0, last_tiles = 0, last_entry_points = 0;
     double last_seconds = 0.0, last_radial_seconds = 0.0;
+    double phase_seconds[7] = {};
+    unsigned long long last_screen_triples = 0, last_screen_steps = 0;
     int min_level = 0;
     double tol = 1e-16;
     int radial_mfma = 0;
@@ -1923,7 +1989,7 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     rc |= dalloc(P, &P->d_OCV, (size_t) ocvlen);
     rc |= dalloc(P, &P->d_delta, (size_t) delta_len);
     rc |= dalloc(P, &P->d_out, (size_t) 4 * op_stride);
-    rc |= dalloc(P, &P->d_counters, 4);
+    rc |= dalloc(P, &P->d_counters, 8);
     if (rc) {
         eng_plan_destroy(P);
         return 1;
@@ -1971,7 +2037,7 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     J.OCV = P->d_OCV;
     J.nshell = nshell;   // lookup tables are indexed by basis shell; outercore shells follow them in sh
     J.op_stride = op_stride;
-    for (int i = 0; i < 4; i++) HIPCHK(hipEventCreate(&P->ev[i]));
+    for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&P->ev[i]));
     {
         // radial quadrature on the matrix cores unless GRPP_RADIAL_KERNEL=valu
         const char *k = getenv("GRPP_RADIAL_KERNEL");
@@ -1984,7 +2050,9 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
 extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
 {
     hipStream_t st = (hipStream_t) stream_;
-    HIPCHK(hipMemsetAsync(P->d_counters, 0, 4 * sizeof(unsigned long long), st));
+    // events: 0 start | tables | 1 | screening | 2 | radial product | 3 | canonical splits | 4 |
+    // projectors, outercore vectors | 5 | type-1 | 6 | contraction | 7
+    HIPCHK(hipMemsetAsync(P->d_counters, 0, 8 * sizeof(unsigned long long), st));
     HIPCHK(hipEventRecord(P->ev[0], st));
     if (P->n_tab > 0) {
         dim3 g(NPTS / 256, P->n_tab);
@@ -1994,11 +2062,12 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
         dim3 g(NPTS / 256, P->n_wpots);
         hipLaunchKernelGGL(k_tab_wU, g, dim3(256), 0, st, P->d_wpots, P->J, P->T, P->d_wU);
     }
-    if (P->n_tiles == 0) HIPCHK(hipEventRecord(P->ev[1], st));
-    if (P->n_tiles > 0) {
+    HIPCHK(hipEventRecord(P->ev[1], st));
+    if (P->n_tiles > 0)
         hipLaunchKernelGGL(k_screen, dim3(4 * P->n_tiles), dim3(64), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->J, P->T,
-                           P->d_scr);
-        HIPCHK(hipEventRecord(P->ev[1], st));
+                           P->d_scr, P->d_counters);
+    HIPCHK(hipEventRecord(P->ev[2], st));
+    if (P->n_tiles > 0) {
         if (P->radial_mfma)
             hipLaunchKernelGGL(k_radial_mfma, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs, P->d_rows,
                                P->d_F, P->T.rpow, P->d_wU, P->d_Q, P->min_level, P->tol, P->d_counters, P->d_scr, P->J.sh);
@@ -2006,15 +2075,17 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
             hipLaunchKernelGGL(k_radial, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->d_F,
                                P->T.rpow, P->d_wU, P->d_Q, P->min_level, P->tol, P->d_counters, P->d_scr, P->J.sh);
     }
+    HIPCHK(hipEventRecord(P->ev[3], st));
     if (P->n_tiles > 0)
         hipLaunchKernelGGL(k_canon, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->J, P->d_Q,
                            P->d_Qc);
-    HIPCHK(hipEventRecord(P->ev[2], st));
+    HIPCHK(hipEventRecord(P->ev[4], st));
     if (P->n_proj > 0) hipLaunchKernelGGL(k_proj, dim3(P->n_proj), dim3(256), 0, st, P->d_proj, P->J, P->T, P->d_P);
     if (P->n_delta_centers > 0)
         hipLaunchKernelGGL(k_delta, dim3(P->n_delta_centers), dim3(64), 0, st, P->d_delta_centers,
                            P->n_delta_centers, P->J, P->d_delta);
     if (P->n_ocv > 0) hipLaunchKernelGGL(k_ocv, dim3(P->n_ocv), dim3(256), 0, st, P->d_ocvi, P->J, P->T, P->d_OCV);
+    HIPCHK(hipEventRecord(P->ev[5], st));
     if (P->n_items > 0) {
         // dynamic LDS beyond the 64 KiB default (up to the 160 KiB of a gfx950 CU) must be opted into
         HIPCHK(hipFuncSetAttribute((const void *) k_type1, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -2023,22 +2094,30 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
                                    (P->ct_qs + P->ct_ts) * (int) sizeof(double)));
         hipLaunchKernelGGL(k_type1, dim3(P->n_items), dim3(256), (size_t) (P->t1_scratch + P->t1_acc) * sizeof(double),
                            st, P->d_items, P->J, P->T, P->d_out, P->parts, P->t1_scratch);
+    }
+    HIPCHK(hipEventRecord(P->ev[6], st));
+    if (P->n_items > 0)
         hipLaunchKernelGGL(k_contract, dim3(P->n_items), dim3(256), (size_t) (P->ct_qs + P->ct_ts) * sizeof(double),
                            st, P->d_items, P->J, P->T, P->d_out, P->parts, P->nm_max, P->ct_qs, P->ct_ts);
-    }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(P->ev[3], st));
-    HIPCHK(hipEventSynchronize(P->ev[3]));
-    float ms = 0, msr = 0;
-    HIPCHK(hipEventElapsedTime(&ms, P->ev[0], P->ev[3]));
-    HIPCHK(hipEventElapsedTime(&msr, P->ev[1], P->ev[2]));
+    HIPCHK(hipEventRecord(P->ev[7], st));
+    HIPCHK(hipEventSynchronize(P->ev[7]));
+    for (int i = 0; i < 7; i++) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, P->ev[i], P->ev[i + 1]));
+        P->phase_seconds[i] = ms * 1e-3;
+    }
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, P->ev[0], P->ev[7]));
     P->last_seconds = ms * 1e-3;
-    P->last_radial_seconds = msr * 1e-3;
-    unsigned long long cnt[4];
+    P->last_radial_seconds = P->phase_seconds[2];
+    unsigned long long cnt[8];
     HIPCHK(hipMemcpy(cnt, P->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
     P->last_points = cnt[0];
     P->last_tiles = cnt[1];
     P->last_entry_points = cnt[2];
+    P->last_screen_triples = cnt[3];
+    P->last_screen_steps = cnt[4];
     return 0;
 }
 
@@ -2054,7 +2133,7 @@ extern "C" void eng_plan_destroy(eng_plan_t *P)
 {
     if (!P) return;
     for (void *b : P->bufs) (void) hipFree(b);
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 8; i++)
         if (P->ev[i]) (void) hipEventDestroy(P->ev[i]);
     delete P;
 }
@@ -2071,6 +2150,18 @@ extern "C" int eng_plan_stats(const eng_plan_t *P, libgrpp_batch_stats_t *s)
     s->fp64_flops = 2.0 * (double) P->last_entry_points;
     s->device_seconds = P->last_seconds;
     s->radial_seconds = P->last_radial_seconds;
+    s->tables_seconds = P->phase_seconds[0];
+    s->screen_seconds = P->phase_seconds[1];
+    s->canon_seconds = P->phase_seconds[3];
+    s->proj_seconds = P->phase_seconds[4];
+    s->type1_seconds = P->phase_seconds[5];
+    s->contract_seconds = P->phase_seconds[6];
+    s->screen_triples = (long long) P->last_screen_triples;
+    s->screen_steps = (long long) P->last_screen_steps;
+    // envelope estimates in FLOP equivalents (K_l evaluation 10, exp and erf 20 each):
+    // per fixed-point evaluation 2 Bessel ratios (2 K + 1 division each) + 5 = 47;
+    // per triple 2 K + exp + erf + 20 (r^n, prefactors, Gaussian exponent, products) = 80
+    s->screen_flops = 80.0 * (double) P->last_screen_triples + 47.0 * (double) P->last_screen_steps;
     return 0;
 }
 

@@ -21,7 +21,10 @@ _lib = None
 class Stats(C.Structure):
     _fields_ = [("n_tasks", C.c_longlong), ("n_integrals", C.c_longlong), ("n_radial_entries", C.c_longlong),
                 ("n_radial_points", C.c_longlong), ("n_fallback_tiles", C.c_longlong), ("n_tiles", C.c_longlong),
-                ("fp64_flops", C.c_double), ("device_seconds", C.c_double), ("radial_seconds", C.c_double)]
+                ("fp64_flops", C.c_double), ("device_seconds", C.c_double), ("radial_seconds", C.c_double),
+                ("tables_seconds", C.c_double), ("screen_seconds", C.c_double), ("canon_seconds", C.c_double),
+                ("proj_seconds", C.c_double), ("type1_seconds", C.c_double), ("contract_seconds", C.c_double),
+                ("screen_triples", C.c_longlong), ("screen_steps", C.c_longlong), ("screen_flops", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
