@@ -6,11 +6,16 @@
 //   k_tab_F     F^{C,A}_lambda(r_i): contracted radial functions of every shell A about every
 //               centre C on the level-ordered Log3 grid (radial_type2_integral.c:465-489);
 //   k_tab_wU    w_i U_p(r_i) for every semilocal / spin-orbit channel p (potential.c:70-81);
+//   k_screen    the Shaw-Hill envelope estimate of every radial entry (screening.c:193-378), one
+//               wave per entry; it steers the quadrature exactly as in the reference;
 //   k_radial    the radial integrals of ALL shell pairs of a centre and channel as one FP64
 //               product Q = G diag(wU) G^T over grid points, rows (shell, lambda, n_A) with
 //               G = r^{n_A} F; the K loop runs level by level (31, 63, ..., 8191 points) and
 //               applies libgrpp's per-entry nested-doubling stop rule at each level boundary
 //               (radial_type2_integral.c:242-318); tiles stop early when every entry stopped;
+//               k_radial_mfma is the same product on v_mfma_f64_16x16x4f64 (the default);
+//   k_canon     every split of (lambda1, lambda2, N) reads one canonical split, as libgrpp keeps
+//               one Q^N per N (type2_integrals.c:183-186);
 //   k_proj      angular projectors P^{C,A,L}[i][m][(lambda,n_A)] = binomial expansion of the
 //               Cartesian component about C times Omega^lambda_{Lm} (type2_integrals.c:117-211,
 //               angular_integrals.c:63-111), and their L-matrix products for spin-orbit;
