@@ -6,9 +6,12 @@ all ordered shell pairs x all GRPP centres) of the ``heavy_cluster`` workload: 8
 12-bohr cube, GRPP with semilocal L=0..4 + local, SO L=1..4, outercore s/p projectors, basis
 4s4p3d2f1g with 6 primitives per shell (112 shells, dim 552, 100,352 shell-pair x centre tasks),
 synthetic, seed 0.  Inputs are packed and resident in HBM before timing; a step runs the kernels.
+The build is timed twice: with the plain adaptive method, then with radial tables ordered by the
+learned level predictor (trained beforehand, untimed, on seeds 1-4); ``value`` is the latter and
+``learned_grid`` holds the comparison.
 
 Prints one JSON line (see the keys below).  --gpus N with N > 1 is not supported: the build is
-single-GPU (tasks shard by shell-pair rows without communication).
+single-GPU (tasks would shard by shell-pair rows without communication).
 """
 from __future__ import annotations
 
@@ -73,6 +76,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="heavy_cluster")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-learned", action="store_true", help="time the plain method only")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device-to-device copy (3 times): the known-size kernel "
@@ -94,24 +98,59 @@ def main():
     from libgrpp_amd import configs, lib as G
 
     sysm = configs.WORKLOADS[args.workload]()
-    plan = G.Plan(sysm, 0)
-    for _ in range(args.warmup):
-        plan.execute()
-    torch.cuda.synchronize()
     phases = ["tables_seconds", "screen_seconds", "radial_seconds", "canon_seconds", "proj_seconds",
               "type1_seconds", "contract_seconds", "device_seconds"]
-    times = {k: [] for k in phases}
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.execute()
-        st = plan.stats()   # per-phase HIP-event times of this step, on the plan's stream
-        for k in phases:
-            times[k].append(st[k])
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    st = plan.stats()
+
+    def timed(plan):
+        """warm-up, then exactly --steps timed builds bracketed by device synchronisation"""
+        for _ in range(args.warmup):
+            plan.execute()
+        torch.cuda.synchronize()
+        times = {k: [] for k in phases}
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.execute()
+            st = plan.stats()   # per-phase HIP-event times of this step, on the plan's stream
+            for k in phases:
+                times[k].append(st[k])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return wall, plan.stats(), {k: float(np.mean(v)) for k, v in times.items()}
+
+    # plain adaptive method
+    plan = G.Plan(sysm, 0)
+    wall_plain, st_plain, mean_plain = timed(plan)
+    plain_out = plan.download()
     plan.close()
-    mean = {k: float(np.mean(v)) for k, v in times.items()}
+    # learned grid levels: the predictor is trained (untimed) on heavy_cluster seeds 1-4, then its
+    # BF16 inference orders the radial tables of the seed-0 workload
+    learned_info = None
+    wall, st, mean = wall_plain, st_plain, mean_plain
+    if not args.no_learned:
+        from libgrpp_amd import learned as LG
+        t0 = time.perf_counter()
+        X, T = LG.training_set(LG.default_training_systems(4))
+        net, loss = LG.train_predictor(X, T, epochs=30)
+        t_train = time.perf_counter() - t0
+        plan = G.Plan(sysm, 0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        LG.apply(plan, net, bf16=True)
+        torch.cuda.synchronize()
+        t_pred = time.perf_counter() - t0
+        wall, st, mean = timed(plan)
+        out = plan.download()
+        plan.close()
+        net.close()
+        diff = max(float(np.abs(a - b).max()) for a, b in zip(out, plain_out))
+        learned_info = {"plain_tasks_per_s": configs.n_tasks(sysm) / (wall_plain / args.steps),
+                        "plain_ms_per_step": 1e3 * wall_plain / args.steps,
+                        "speedup_vs_plain": wall_plain / wall,
+                        "radial_points_plain": st_plain["n_radial_points"], "radial_points_learned": st["n_radial_points"],
+                        "radial_ms_plain": 1e3 * mean_plain["radial_seconds"],
+                        "radial_ms_learned": 1e3 * mean["radial_seconds"],
+                        "max_abs_diff_vs_plain": diff, "training_rows": int(X.shape[0]),
+                        "training_loss": loss, "training_s": t_train, "inference_and_reorder_ms": 1e3 * t_pred}
 
     ntask = configs.n_tasks(sysm)
     nint = configs.n_integrals(sysm)
@@ -122,7 +161,8 @@ def main():
     rad = mean["radial_seconds"]
     rad_achieved = st["fp64_flops"] / rad / 1e12
     line = {
-        "metric": "FP64 GRPP shell-pair x centre tasks/s (full GRPP matrix build, plain adaptive radial grid)",
+        "metric": ("FP64 GRPP shell-pair x centre tasks/s: learned-grid vs plain GPU (full GRPP matrix build)"
+                   if learned_info else "FP64 GRPP shell-pair x centre tasks/s: plain GPU (full GRPP matrix build)"),
         "value": tasks_per_s,
         "unit": "tasks/s",
         "n_gpus": 1,
@@ -152,6 +192,8 @@ def main():
                                         "flops": st["fp64_flops"], "points": st["n_radial_points"],
                                         "tiles": st["n_tiles"], "traffic": traffic_bytes("k_radial_mfma")}},
     }
+    if learned_info:
+        line["learned_grid"] = learned_info
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(sysm, args.cpu_budget)
     print(json.dumps(line))

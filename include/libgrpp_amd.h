@@ -24,7 +24,8 @@ enum {
     LIBGRPP_BATCH_DEFAULT = 0,
     LIBGRPP_BATCH_LEARNED_GRID = 1,   /* predicted fixed radial grids + embedded-rule check + adaptive fallback */
     LIBGRPP_BATCH_GRADIENT = 2,       /* plan also evaluates nuclear gradients (see ..._gradients) */
-    LIBGRPP_BATCH_NO_SYMMETRY = 4     /* evaluate every ordered pair instead of mirroring B,A from A,B */
+    LIBGRPP_BATCH_NO_SYMMETRY = 4,    /* evaluate every ordered pair instead of mirroring B,A from A,B */
+    LIBGRPP_BATCH_RECORD_LEVELS = 8   /* plan records each radial entry's stopping level (training labels) */
 };
 
 typedef struct {
@@ -83,9 +84,38 @@ int libgrpp_batch_plan_execute(libgrpp_batch_plan_t *plan, void *stream);
 int libgrpp_batch_plan_download(libgrpp_batch_plan_t *plan, double *arep, double *so_x, double *so_y, double *so_z);
 int libgrpp_batch_plan_dim(const libgrpp_batch_plan_t *plan);
 int libgrpp_batch_plan_stats(const libgrpp_batch_plan_t *plan, libgrpp_batch_stats_t *stats);
+/*
+ * Radial-grid level predictor hooks.  The plan's radial tables have one row per (centre, channel
+ * L, shell, lambda, n_A).  features: *n rows x 12 floats (L, l, lambda, n, |CA|, ln alpha_min,
+ * ln alpha_max, ln zeta_min, ln zeta_max, ln(1 + 2 alpha_max |CA|), primitives, on-centre flag);
+ * features == NULL returns *n only.  row_levels (plans created with LIBGRPP_BATCH_RECORD_LEVELS,
+ * after an execute): *n rows x 3 floats = max, mean and fraction >= 7 of the stopping levels of
+ * the row's entries (level k = 32 * 2^k - 1 Log3 points).  set_predictions: n rows x 3 floats in
+ * the same layout; the plan orders each radial table's rows by the first value so that tiles hold
+ * entries of like level.  Every entry still stops by the reference's rule, so results do not
+ * depend on the predictions.
+ */
 int libgrpp_batch_plan_set_predictions(libgrpp_batch_plan_t *plan, int n, const float *level_and_interval);
 int libgrpp_batch_plan_features(libgrpp_batch_plan_t *plan, int *n, float *features);
+int libgrpp_batch_plan_row_levels(libgrpp_batch_plan_t *plan, int *n, float *levels);
 void libgrpp_batch_plan_destroy(libgrpp_batch_plan_t *plan);
+
+/*
+ * Radial-grid level predictor: a 12 -> 128 -> 128 -> 128 -> 3 SiLU multilayer perceptron trained
+ * and evaluated by HIP kernels (FP32 training with Adam; inference in FP32 or with BF16 weights).
+ * Inputs are the 12 row features of libgrpp_batch_plan_features, outputs the 3 level statistics
+ * of libgrpp_batch_plan_row_levels.  Host arrays: X n x 12, T and Y n x 3 (row-major floats).
+ */
+typedef struct libgrpp_mlp libgrpp_mlp_t;
+int libgrpp_mlp_nparams(void);
+int libgrpp_mlp_create(unsigned seed, libgrpp_mlp_t **mlp);
+int libgrpp_mlp_set_normalisation(libgrpp_mlp_t *mlp, const float *mean, const float *scale);
+int libgrpp_mlp_params(libgrpp_mlp_t *mlp, float *get, const float *set);
+int libgrpp_mlp_gradient(libgrpp_mlp_t *mlp, int n, const float *X, const float *T, float *grad, float *loss);
+int libgrpp_mlp_train(libgrpp_mlp_t *mlp, int n, const float *X, const float *T, int epochs, int batch, float lr,
+                      unsigned seed, float *loss);
+int libgrpp_mlp_predict(libgrpp_mlp_t *mlp, int n, const float *X, float *Y, int bf16);
+void libgrpp_mlp_destroy(libgrpp_mlp_t *mlp);
 
 /* thread-local description of the last error of any libgrpp_amd call */
 const char *libgrpp_last_error(void);

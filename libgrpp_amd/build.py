@@ -39,6 +39,8 @@ def build(verbose: bool = False) -> str:
     steps = [
         ([HIPCC] + HIP_FLAGS + inc + ["-c", os.path.join(CSRC, "engine.hip"), "-o", os.path.join(BUILD, "engine.o")],
          os.path.join(BUILD, "engine.o")),
+        ([HIPCC] + HIP_FLAGS + inc + ["-c", os.path.join(CSRC, "mlp.hip"), "-o", os.path.join(BUILD, "mlp.o")],
+         os.path.join(BUILD, "mlp.o")),
         (["g++", "-O3", "-fPIC", "-std=c++17"] + inc + ["-c", os.path.join(CSRC, "tables.cpp"), "-o",
                                                          os.path.join(BUILD, "tables.o")], os.path.join(BUILD, "tables.o")),
         (["gcc", "-O3", "-fPIC", "-std=gnu11", "-Wall"] + inc + ["-c", os.path.join(CSRC, "api.c"), "-o",

@@ -340,8 +340,9 @@ int libgrpp_batch_plan_create(int nshells, libgrpp_shell_t *const *shells, int n
         }
     eng_plan_t *eng = NULL;
     long long n2 = (long long) dim * dim;
-    int rc = eng_plan_create(nshells, shells, ncenters, cs, nidx, cidx, nit, it, n2, n2, ENG_ALL,
-                             (flags & LIBGRPP_BATCH_LEARNED_GRID) ? ENG_F_LEARNED : 0, &eng);
+    int eflags = ((flags & LIBGRPP_BATCH_LEARNED_GRID) ? ENG_F_LEARNED : 0) |
+                 ((flags & LIBGRPP_BATCH_RECORD_LEVELS) ? ENG_F_LEVELS : 0);
+    int rc = eng_plan_create(nshells, shells, ncenters, cs, nidx, cidx, nit, it, n2, n2, ENG_ALL, eflags, &eng);
     free(off);
     free(cidx);
     free(cs);
@@ -375,6 +376,7 @@ int libgrpp_batch_plan_download(libgrpp_batch_plan_t *p, double *arep, double *s
 int libgrpp_batch_plan_dim(const libgrpp_batch_plan_t *p) { return p->dim; }
 int libgrpp_batch_plan_stats(const libgrpp_batch_plan_t *p, libgrpp_batch_stats_t *s) { return eng_plan_stats(p->eng, s); }
 int libgrpp_batch_plan_features(libgrpp_batch_plan_t *p, int *n, float *f) { return eng_plan_features(p->eng, n, f); }
+int libgrpp_batch_plan_row_levels(libgrpp_batch_plan_t *p, int *n, float *v) { return eng_plan_row_levels(p->eng, n, v); }
 int libgrpp_batch_plan_set_predictions(libgrpp_batch_plan_t *p, int n, const float *v)
 {
     return eng_plan_set_predictions(p->eng, n, v);

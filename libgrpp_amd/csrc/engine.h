@@ -42,7 +42,8 @@ typedef struct {
 typedef struct eng_plan eng_plan_t;
 
 /* flags */
-enum { ENG_F_LEARNED = 1, ENG_F_STATS = 2 };
+enum { ENG_F_LEARNED = 1, ENG_F_STATS = 2, ENG_F_LEVELS = 4 };
+#define ENG_NFEAT 12   /* level-predictor features per radial row */
 
 int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int ncenter, const eng_center_t *centers,
                     int nidx, const int *center_index, int nitem, const eng_item_t *items, long long out_len,
@@ -54,6 +55,7 @@ void eng_plan_destroy(eng_plan_t *plan);
 int eng_plan_stats(const eng_plan_t *plan, libgrpp_batch_stats_t *stats);
 int eng_plan_features(eng_plan_t *plan, int *n, float *features);
 int eng_plan_set_predictions(eng_plan_t *plan, int n, const float *pred);
+int eng_plan_row_levels(eng_plan_t *plan, int *n, float *levels);
 
 /* convenience: create + execute + download + destroy */
 int eng_run(int nshell, libgrpp_shell_t *const *shells, int ncenter, const eng_center_t *centers, int nidx,

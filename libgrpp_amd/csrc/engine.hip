@@ -85,7 +85,7 @@ struct DCenter {
 };
 
 struct DPot {
-    int L, J, kind, nterm, toff, slot, M, pad;
+    int L, J, kind, nterm, toff, slot, M, rbase;   // rbase: first row of the slot
     long long qoff;
     double scale, thr;
 };
@@ -96,7 +96,7 @@ struct TabItem {
     int c, A, frow, lam;
 };
 struct RowDesc {
-    int frow, n, shell, lam;
+    int frow, n, shell, lam, lidx;   // lidx: logical index in the slot (shell, n, lambda order)
 };
 struct QJob {
     int rows, M, pot, maxlev;
@@ -137,6 +137,7 @@ struct DevJob {
     const OcEntry *oce;
     // lookup tables, slot = (centre, L) pair
     const int *slot_rowoff;        // [slot * nshell + A]
+    const int *rowpos;             // [slot row base + logical row] -> physical row of the radial table
     const long long *slot_poff;    // [slot * nshell + A]
     const long long *slot_ploff;   // [slot * nshell + A] (-1 if the slot has no SO channel)
     const long long *slot_pb2;     // bra projector, type-2 thresholds
@@ -293,15 +294,26 @@ constexpr int RK = 32;   // grid points per K step
 
 // ================================================================== Shaw-Hill screening of the radial entries
 // flat index of the canonical split of entry (row, col) of a radial table
-__device__ __forceinline__ long long canon_index(const RowDesc *R, const DShell *sh, const QJob &jb, int row, int col)
+// entry (row a, column b) is the canonical split of its (lambda1, lambda2, N): the row carries
+// the smallest valid n_A when its (shell, lambda) key is the smaller one, else the largest
+__device__ __forceinline__ bool is_canonical(int L, int la, int lb, const RowDesc &ra, const RowDesc &rb)
+{
+    const int N = ra.n + rb.n;
+    const bool a_first = (ra.shell < rb.shell) || (ra.shell == rb.shell && ra.lam <= rb.lam);
+    const int cn = a_first ? dev_canon_nA(L, la, lb, ra.lam, rb.lam, N) : dev_canon_nA_max(L, la, lb, ra.lam, rb.lam, N);
+    return cn == ra.n;
+}
+
+__device__ __forceinline__ long long canon_index(const RowDesc *R, const DShell *sh, const int *pos, const QJob &jb,
+                                                int row, int col)
 {
     const RowDesc ra = R[row], rb = R[col];
     const int L = jb.L, la = sh[ra.shell].L, lb = sh[rb.shell].L, N = ra.n + rb.n;
     const bool a_first = (ra.shell < rb.shell) || (ra.shell == rb.shell && ra.lam <= rb.lam);
     const int cn = a_first ? dev_canon_nA(L, la, lb, ra.lam, rb.lam, N) : dev_canon_nA_max(L, la, lb, ra.lam, rb.lam, N);
-    const int r2 = row - dev_kcol(L, ra.n, ra.lam) + dev_kcol(L, cn, ra.lam);
-    const int c2 = col - dev_kcol(L, rb.n, rb.lam) + dev_kcol(L, N - cn, rb.lam);
-    return (long long) r2 * jb.M + c2;
+    const int r2 = ra.lidx - dev_kcol(L, ra.n, ra.lam) + dev_kcol(L, cn, ra.lam);
+    const int c2 = rb.lidx - dev_kcol(L, rb.n, rb.lam) + dev_kcol(L, N - cn, rb.lam);
+    return (long long) pos[jb.rows + r2] * jb.M + pos[jb.rows + c2];
 }
 
 
@@ -472,11 +484,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
         if (t.ti == t.tj && row > col) continue;
         const RowDesc ra = R[row], rb = R[col];
         const DShell A = J.sh[ra.shell], B = J.sh[rb.shell];
-        const int N = ra.n + rb.n;
-        const bool a_first = (ra.shell < rb.shell) || (ra.shell == rb.shell && ra.lam <= rb.lam);
-        const int cn = a_first ? dev_canon_nA(jb.L, A.L, B.L, ra.lam, rb.lam, N)
-                               : dev_canon_nA_max(jb.L, A.L, B.L, ra.lam, rb.lam, N);
-        if (cn != ra.n) continue;
+        // the canonical split of (lambda1, lambda2, N) may sit on either side of the diagonal
+        // (rows need not be in (shell, n) order), so both orientations are tested
+        if (!is_canonical(jb.L, A.L, B.L, ra, rb) && !is_canonical(jb.L, B.L, A.L, rb, ra)) continue;
         const bool a_on = (c.x - A.x) * (c.x - A.x) + (c.y - A.y) * (c.y - A.y) + (c.z - A.z) * (c.z - A.z) == 0.0;
         const bool b_on = (c.x - B.x) * (c.x - B.x) + (c.y - B.y) * (c.y - B.y) + (c.z - B.z) * (c.z - B.z) == 0.0;
         const bool zero = (a_on && ra.lam >= 1) || (b_on && rb.lam >= 1);
@@ -555,9 +565,10 @@ __global__ __launch_bounds__(256) void k_canon(const TileItem *__restrict__ tile
         const int la = J.sh[ra.shell].L, lb = J.sh[rb.shell].L, N = ra.n + rb.n;
         const bool a_first = (ra.shell < rb.shell) || (ra.shell == rb.shell && ra.lam <= rb.lam);
         const int cn = a_first ? dev_canon_nA(L, la, lb, ra.lam, rb.lam, N) : dev_canon_nA_max(L, la, lb, ra.lam, rb.lam, N);
-        const int r2 = row - dev_kcol(L, ra.n, ra.lam) + dev_kcol(L, cn, ra.lam);
-        const int c2 = col - dev_kcol(L, rb.n, rb.lam) + dev_kcol(L, N - cn, rb.lam);
-        Qc[jb.qoff + (long long) row * jb.M + col] = Q[jb.qoff + (long long) r2 * jb.M + c2];
+        const int r2 = ra.lidx - dev_kcol(L, ra.n, ra.lam) + dev_kcol(L, cn, ra.lam);
+        const int c2 = rb.lidx - dev_kcol(L, rb.n, rb.lam) + dev_kcol(L, N - cn, rb.lam);
+        Qc[jb.qoff + (long long) row * jb.M + col] =
+            Q[jb.qoff + (long long) J.rowpos[jb.rows + r2] * jb.M + J.rowpos[jb.rows + c2]];
     }
 }
 
@@ -568,7 +579,8 @@ __global__ __launch_bounds__(256) void k_radial(const TileItem *__restrict__ til
                                                 const double *__restrict__ rpow, const double *__restrict__ wU,
                                                 double *__restrict__ Q, int min_level, double tol,
                                                 unsigned long long *__restrict__ counters,
-                                                const double *__restrict__ scr, const DShell *__restrict__ J_sh)
+                                                const double *__restrict__ scr, const DShell *__restrict__ J_sh,
+                                                const int *__restrict__ pos, signed char *__restrict__ lev_out)
 {
     __shared__ double As[RK][RT + 2];
     __shared__ double Bs[RK][RT + 2];
@@ -592,22 +604,25 @@ __global__ __launch_bounds__(256) void k_radial(const TileItem *__restrict__ til
 
     double acc[4][4], S[4][4], res[4][4], est[4][4];
     unsigned done = 0;
+    int stop[4][4];   // stopping level per entry: -1 screened out, 9 not converged by the last level
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             S[a][b] = 0.0;
             res[a][b] = 0.0;
+            stop[a][b] = 9;
             est[a][b] = __longlong_as_double(0x7ff8000000000000LL);
             const int row = t.ti * RT + ty * 4 + a, col = t.tj * RT + tx * 4 + b;
             if (row >= jb.M || col >= jb.M) {
                 done |= 1u << (a * 4 + b);
             } else if (scr) {
                 // screened below the tolerance: the envelope estimate is the result (radial_type2_integral.c:277-280)
-                est[a][b] = scr[jb.qoff + canon_index(R, J_sh, jb, row, col)];
+                est[a][b] = scr[jb.qoff + canon_index(R, J_sh, pos, jb, row, col)];
                 if (!isnan(est[a][b]) && fabs(est[a][b]) < tol) {
                     done |= 1u << (a * 4 + b);
                     res[a][b] = est[a][b];
+                    stop[a][b] = -1;
                 }
             }
         }
@@ -659,6 +674,7 @@ __global__ __launch_bounds__(256) void k_radial(const TileItem *__restrict__ til
                     if (!(done & bit) && !forced && lev >= min_level && fabs(S[a][b] - prev) <= tol) {
                         done |= bit;
                         res[a][b] = S[a][b];
+                        stop[a][b] = lev;
                     }
                 }
             }
@@ -686,6 +702,10 @@ __global__ __launch_bounds__(256) void k_radial(const TileItem *__restrict__ til
             if (t.ti < t.tj || row <= col) {
                 Q[jb.qoff + (long long) row * jb.M + col] = v;
                 Q[jb.qoff + (long long) col * jb.M + row] = v;
+                if (lev_out) {
+                    lev_out[jb.qoff + (long long) row * jb.M + col] = (signed char) stop[a][b];
+                    lev_out[jb.qoff + (long long) col * jb.M + row] = (signed char) stop[a][b];
+                }
             }
         }
 }
@@ -695,12 +715,14 @@ __global__ __launch_bounds__(256) void k_radial(const TileItem *__restrict__ til
 // in element r of lane l, as measured on gfx950); the per-entry stop rule is unchanged.
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+template <bool REC>
 __global__ __launch_bounds__(256) void k_radial_mfma(const TileItem *__restrict__ tiles, const QJob *__restrict__ jobs,
                                                      const RowDesc *__restrict__ rows, const double *__restrict__ F,
                                                      const double *__restrict__ rpow, const double *__restrict__ wU,
                                                      double *__restrict__ Q, int min_level, double tol,
                                                      unsigned long long *__restrict__ counters,
-                                                     const double *__restrict__ scr, const DShell *__restrict__ J_sh)
+                                                     const double *__restrict__ scr, const DShell *__restrict__ J_sh,
+                                                     const int *__restrict__ pos, signed char *__restrict__ lev_out)
 {
     __shared__ double As[RK][RT + 2];
     __shared__ double Bs[RK][RT + 2];
@@ -727,6 +749,7 @@ __global__ __launch_bounds__(256) void k_radial_mfma(const TileItem *__restrict_
     v4d acc[2][2];
     double S[2][2][4], res[2][2][4], est[2][2][4];
     unsigned done = 0;
+    int stop[2][2][4];   // stopping level per entry: -1 screened out, 9 not converged by the last level
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -735,15 +758,17 @@ __global__ __launch_bounds__(256) void k_radial_mfma(const TileItem *__restrict_
             for (int r = 0; r < 4; r++) {
                 S[a][b][r] = 0.0;
                 res[a][b][r] = 0.0;
+                if (REC) stop[a][b][r] = 9;
                 est[a][b][r] = __longlong_as_double(0x7ff8000000000000LL);
                 const int row = t.ti * RT + wr + a * 16 + lk + 4 * r, col = t.tj * RT + wc + b * 16 + li;
                 if (row >= jb.M || col >= jb.M) {
                     done |= 1u << (a * 8 + b * 4 + r);
                 } else if (scr) {
-                    est[a][b][r] = scr[jb.qoff + canon_index(R, J_sh, jb, row, col)];
+                    est[a][b][r] = scr[jb.qoff + canon_index(R, J_sh, pos, jb, row, col)];
                     if (!isnan(est[a][b][r]) && fabs(est[a][b][r]) < tol) {
                         done |= 1u << (a * 8 + b * 4 + r);
                         res[a][b][r] = est[a][b][r];
+                        if (REC) stop[a][b][r] = -1;
                     }
                 }
             }
@@ -793,6 +818,7 @@ __global__ __launch_bounds__(256) void k_radial_mfma(const TileItem *__restrict_
                         if (!(done & bit) && !forced && lev >= min_level && fabs(S[a][b][r] - prev) <= tol) {
                             done |= bit;
                             res[a][b][r] = S[a][b][r];
+                            if (REC) stop[a][b][r] = lev;
                         }
                     }
                 }
@@ -821,6 +847,10 @@ __global__ __launch_bounds__(256) void k_radial_mfma(const TileItem *__restrict_
                 if (t.ti < t.tj || row <= col) {
                     Q[jb.qoff + (long long) row * jb.M + col] = v;
                     Q[jb.qoff + (long long) col * jb.M + row] = v;
+                    if (REC) {
+                        lev_out[jb.qoff + (long long) row * jb.M + col] = (signed char) stop[a][b][r];
+                        lev_out[jb.qoff + (long long) col * jb.M + row] = (signed char) stop[a][b][r];
+                    }
                 }
             }
 }
@@ -1344,7 +1374,7 @@ __global__ __launch_bounds__(256) void k_contract(const WorkItem *__restrict__ i
                     const int rb = J.slot_rowoff[(long long) p.slot * J.nshell + it.b];
                     for (int e = tid; e < KA * KB; e += blockDim.x) {
                         const int k1 = e / KB, k2 = e % KB;
-                        Qs[e] = J.Q[p.qoff + (long long) (ra + k1) * p.M + rb + k2];
+                        Qs[e] = J.Q[p.qoff + (long long) J.rowpos[p.rbase + ra + k1] * p.M + J.rowpos[p.rbase + rb + k2]];
                     }
                 }
                 __syncthreads();
@@ -1607,6 +1637,16 @@ struct eng_plan {
     int nm_max = 1;
     int ct_qs = 1, ct_ts = 1;   // k_contract LDS doubles
     int t1_scratch = 64, t1_acc = 1;   // k_type1 LDS doubles
+    // radial rows: logical order per slot (shell, n, lambda), their physical order in the tables,
+    // per-row features for the level predictor, and the recorded stopping levels
+    std::vector<RowDesc> rows_logical;
+    std::vector<int> slot_rows, slot_M;
+    std::vector<int> rowpos;           // [slot row base + logical] -> physical
+    std::vector<QJob> jobs;
+    std::vector<float> features;       // ENG_NFEAT per logical row
+    int *d_rowpos = nullptr;
+    signed char *d_lev = nullptr;      // stopping level per radial entry (ENG_F_LEVELS)
+    long long qlen = 0;
 };
 
 template <class T>
@@ -1815,6 +1855,7 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
                     r.n = n;
                     r.shell = A;
                     r.lam = lam_lo(L, n) + 2 * j;
+                    r.lidx = M;
                     rows.push_back(r);
                     M++;
                 }
@@ -1864,6 +1905,7 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
         q.L = p.L;
         p.qoff = qlen;
         p.M = q.M;
+        p.rbase = q.rows;
         qlen += (long long) q.M * q.M;
         wpots.push_back(k);
         const int nt = (q.M + RT - 1) / RT;
@@ -1995,6 +2037,10 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     rc |= dalloc(P, &P->d_delta, (size_t) delta_len);
     rc |= dalloc(P, &P->d_out, (size_t) 4 * op_stride);
     rc |= dalloc(P, &P->d_counters, 8);
+    std::vector<int> rowpos(rows.size());
+    for (size_t i = 0; i < rows.size(); i++) rowpos[i] = rows[i].lidx;   // identity within each slot
+    rc |= dput(P, &P->d_rowpos, rowpos);
+    if (flags & ENG_F_LEVELS) rc |= dalloc(P, &P->d_lev, (size_t) qlen);
     if (rc) {
         eng_plan_destroy(P);
         return 1;
@@ -2007,6 +2053,49 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     P->n_delta_centers = (int) delta_centers.size();
     P->n_items = nitem;
     P->nm_max = 2 * Lmax_pot + 1;
+    P->rows_logical = rows;
+    P->slot_rows = slot_rows;
+    P->slot_M = slot_M;
+    P->rowpos = rowpos;
+    P->jobs = jobs;
+    P->qlen = qlen;
+    {
+        // level-predictor features of every radial row (centre C, channel L, shell A, lambda, n)
+        std::vector<double> zmin(nslot, 1e300), zmax(nslot, 0.0);
+        for (const DPot &d : pots) {
+            if (d.slot < 0) continue;
+            for (int k = d.toff; k < d.toff + d.nterm; k++) {
+                zmin[d.slot] = std::min(zmin[d.slot], tzeta[k]);
+                zmax[d.slot] = std::max(zmax[d.slot], tzeta[k]);
+            }
+        }
+        P->features.assign(rows.size() * ENG_NFEAT, 0.0f);
+        for (int sl = 0; sl < nslot; sl++)
+            for (int i = 0; i < slot_M[sl]; i++) {
+                const RowDesc &r = rows[(size_t) slot_rows[sl] + i];
+                const DShell &A = sh[r.shell];
+                const DCenter &c = ctr[slots[sl].c];
+                const double CA = std::sqrt((c.x - A.x) * (c.x - A.x) + (c.y - A.y) * (c.y - A.y) + (c.z - A.z) * (c.z - A.z));
+                double amin = 1e300, amax = 0.0;
+                for (int k = A.poff; k < A.poff + A.nprim; k++) {
+                    amin = std::min(amin, alpha[k]);
+                    amax = std::max(amax, alpha[k]);
+                }
+                float *f = &P->features[((size_t) slot_rows[sl] + i) * ENG_NFEAT];
+                f[0] = (float) slots[sl].L;
+                f[1] = (float) A.L;
+                f[2] = (float) r.lam;
+                f[3] = (float) r.n;
+                f[4] = (float) CA;
+                f[5] = (float) std::log(amin);
+                f[6] = (float) std::log(amax);
+                f[7] = (float) std::log(zmin[sl] < 1e299 ? zmin[sl] : 1.0);
+                f[8] = (float) std::log(zmax[sl] > 0.0 ? zmax[sl] : 1.0);
+                f[9] = (float) std::log1p(2.0 * amax * CA);
+                f[10] = (float) A.nprim;
+                f[11] = (CA < 1e-7) ? 1.0f : 0.0f;
+            }
+    }
     {
         // LDS of k_contract: the largest (lambda, n) block of one shell and the (m, i, k2) intermediate
         int kmax = 1, ncmax = 1;
@@ -2026,6 +2115,7 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     J.ctr = d_ctr;
     J.pot = d_pot;
     J.tpow = d_tpow;
+    J.rowpos = P->d_rowpos;
     J.tcoef = d_tcoef;
     J.tzeta = d_tzeta;
     J.cidx = d_cidx;
@@ -2073,12 +2163,18 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
                            P->d_scr, P->d_counters);
     HIPCHK(hipEventRecord(P->ev[2], st));
     if (P->n_tiles > 0) {
-        if (P->radial_mfma)
-            hipLaunchKernelGGL(k_radial_mfma, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs, P->d_rows,
-                               P->d_F, P->T.rpow, P->d_wU, P->d_Q, P->min_level, P->tol, P->d_counters, P->d_scr, P->J.sh);
+        if (P->radial_mfma && P->d_lev)
+            hipLaunchKernelGGL(k_radial_mfma<true>, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs,
+                               P->d_rows, P->d_F, P->T.rpow, P->d_wU, P->d_Q, P->min_level, P->tol, P->d_counters,
+                               P->d_scr, P->J.sh, P->d_rowpos, P->d_lev);
+        else if (P->radial_mfma)
+            hipLaunchKernelGGL(k_radial_mfma<false>, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs,
+                               P->d_rows, P->d_F, P->T.rpow, P->d_wU, P->d_Q, P->min_level, P->tol, P->d_counters,
+                               P->d_scr, P->J.sh, P->d_rowpos, P->d_lev);
         else
             hipLaunchKernelGGL(k_radial, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->d_F,
-                               P->T.rpow, P->d_wU, P->d_Q, P->min_level, P->tol, P->d_counters, P->d_scr, P->J.sh);
+                               P->T.rpow, P->d_wU, P->d_Q, P->min_level, P->tol, P->d_counters, P->d_scr, P->J.sh,
+                               P->d_rowpos, P->d_lev);
     }
     HIPCHK(hipEventRecord(P->ev[3], st));
     if (P->n_tiles > 0)
@@ -2170,13 +2266,92 @@ extern "C" int eng_plan_stats(const eng_plan_t *P, libgrpp_batch_stats_t *s)
     return 0;
 }
 
-extern "C" int eng_plan_features(eng_plan_t *, int *n, float *)
+extern "C" int eng_plan_features(eng_plan_t *P, int *n, float *features)
 {
-    *n = 0;
+    *n = (int) P->rows_logical.size();
+    if (features) memcpy(features, P->features.data(), P->features.size() * sizeof(float));
     return 0;
 }
 
-extern "C" int eng_plan_set_predictions(eng_plan_t *, int, const float *) { return 0; }
+// Per logical row: [max, mean, fraction >= 7] of the stopping levels of its entries over every
+// job of its slot (screened entries count as level 0, not converged as 8), from the last execute.
+extern "C" int eng_plan_row_levels(eng_plan_t *P, int *n, float *out)
+{
+    *n = (int) P->rows_logical.size();
+    if (!out) return 0;
+    if (!P->d_lev) {
+        eng_set_error("plan was created without level recording");
+        return 2;
+    }
+    std::vector<signed char> lev((size_t) P->qlen);
+    HIPCHK(hipMemcpy(lev.data(), P->d_lev, lev.size(), hipMemcpyDeviceToHost));
+    const size_t nr = P->rows_logical.size();
+    std::vector<int> logical_of(nr);   // physical -> logical, per slot
+    for (size_t sl = 0; sl < P->slot_rows.size(); sl++)
+        for (int i = 0; i < P->slot_M[sl]; i++)
+            logical_of[(size_t) P->slot_rows[sl] + P->rowpos[(size_t) P->slot_rows[sl] + i]] = i;
+    std::vector<double> mx(nr, 0.0), sum(nr, 0.0), hi(nr, 0.0), cnt(nr, 0.0);
+    for (const QJob &q : P->jobs)
+        for (int pr = 0; pr < q.M; pr++) {
+            const size_t gi = (size_t) q.rows + logical_of[(size_t) q.rows + pr];
+            for (int pc = 0; pc < q.M; pc++) {
+                int v = lev[(size_t) q.qoff + (size_t) pr * q.M + pc];
+                v = v < 0 ? 0 : (v > 8 ? 8 : v);
+                mx[gi] = std::max(mx[gi], (double) v);
+                sum[gi] += v;
+                hi[gi] += (v >= 7);
+                cnt[gi] += 1.0;
+            }
+        }
+    for (size_t i = 0; i < nr; i++) {
+        out[3 * i] = (float) mx[i];
+        out[3 * i + 1] = (float) (cnt[i] > 0 ? sum[i] / cnt[i] : 0.0);
+        out[3 * i + 2] = (float) (cnt[i] > 0 ? hi[i] / cnt[i] : 0.0);
+    }
+    return 0;
+}
+
+// Orders the rows of every radial table by predicted stopping level (pred[3*i] for logical row
+// i, highest first), so that 64x64 tiles hold entries of like level and stop together.  Results
+// do not depend on the order: every entry keeps the reference's stopping rule.
+extern "C" int eng_plan_set_predictions(eng_plan_t *P, int n, const float *pred)
+{
+    if (n != (int) P->rows_logical.size()) {
+        eng_set_error("set_predictions: %d rows given, plan has %zu", n, P->rows_logical.size());
+        return 2;
+    }
+    std::vector<RowDesc> phys(P->rows_logical.size());
+    for (size_t sl = 0; sl < P->slot_rows.size(); sl++) {
+        const int base = P->slot_rows[sl], M = P->slot_M[sl];
+        std::vector<int> order(M);
+        for (int i = 0; i < M; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return pred[3 * (base + a)] > pred[3 * (base + b)]; });
+        for (int pos = 0; pos < M; pos++) {
+            phys[(size_t) base + pos] = P->rows_logical[(size_t) base + order[pos]];
+            P->rowpos[(size_t) base + order[pos]] = pos;
+        }
+    }
+    HIPCHK(hipMemcpy(P->d_rows, phys.data(), phys.size() * sizeof(RowDesc), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(P->d_rowpos, P->rowpos.data(), P->rowpos.size() * sizeof(int), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// diagnostics: copies a radial buffer of the last execute (0 = Q, 1 = canonical Q, 2 = screening
+// estimates) and the row permutation (logical -> physical) to the host
+extern "C" int eng_plan_debug_table(eng_plan_t *P, int which, double *out, int *rowpos, long long *qoff, int *M,
+                                    int *rows)
+{
+    const double *src = which == 0 ? P->d_Q : (which == 1 ? P->d_Qc : P->d_scr);
+    HIPCHK(hipMemcpy(out, src, (size_t) P->qlen * sizeof(double), hipMemcpyDeviceToHost));
+    memcpy(rowpos, P->rowpos.data(), P->rowpos.size() * sizeof(int));
+    for (size_t j = 0; j < P->jobs.size(); j++) {
+        qoff[j] = P->jobs[j].qoff;
+        M[j] = P->jobs[j].M;
+        rows[j] = P->jobs[j].rows;
+    }
+    return (int) P->jobs.size();
+}
 
 extern "C" int eng_run(int nshell, libgrpp_shell_t *const *shells, int ncenter, const eng_center_t *centers, int nidx,
                        const int *center_index, int nitem, const eng_item_t *items, long long out_len,

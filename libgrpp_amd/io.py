@@ -274,7 +274,7 @@ def read_lower_triangle(path: str) -> np.ndarray:
 
 
 def parity_error(got: np.ndarray, ref: np.ndarray) -> Tuple[float, float, bool]:
-    """BASELINE rule: |d| <= 1e-10, or |d|/|ref| <= 1e-9 where |ref| > 1.
+    """Parity rule: |d| <= 1e-10, or |d|/|ref| <= 1e-9 where |ref| > 1.
     Returns (max abs error, max rel error over |ref|>1, ok)."""
     d = np.abs(got - ref)
     big = np.abs(ref) > 1.0

@@ -33,6 +33,8 @@ class Stats(C.Structure):
 BATCH_LEARNED_GRID = 1
 BATCH_GRADIENT = 2
 BATCH_NO_SYMMETRY = 4
+BATCH_RECORD_LEVELS = 8
+NFEAT = 12
 
 
 def lib() -> C.CDLL:
@@ -167,6 +169,29 @@ class Plan:
         st = Stats()
         lib().libgrpp_batch_plan_stats(self.h, C.byref(st))
         return st.as_dict()
+
+    # ---- radial-grid level predictor hooks
+    def features(self) -> np.ndarray:
+        n = C.c_int()
+        lib().libgrpp_batch_plan_features(self.h, C.byref(n), None)
+        f = np.zeros((n.value, NFEAT), dtype=np.float32)
+        _check(lib().libgrpp_batch_plan_features(self.h, C.byref(n), f.ctypes.data_as(C.POINTER(C.c_float))),
+               "libgrpp_batch_plan_features")
+        return f
+
+    def row_levels(self) -> np.ndarray:
+        n = C.c_int()
+        lib().libgrpp_batch_plan_row_levels(self.h, C.byref(n), None)
+        v = np.zeros((n.value, 3), dtype=np.float32)
+        _check(lib().libgrpp_batch_plan_row_levels(self.h, C.byref(n), v.ctypes.data_as(C.POINTER(C.c_float))),
+               "libgrpp_batch_plan_row_levels")
+        return v
+
+    def set_predictions(self, pred: np.ndarray):
+        pred = np.ascontiguousarray(pred, dtype=np.float32).reshape(-1, 3)
+        _check(lib().libgrpp_batch_plan_set_predictions(self.h, pred.shape[0],
+                                                        pred.ctypes.data_as(C.POINTER(C.c_float))),
+               "libgrpp_batch_plan_set_predictions")
 
     def close(self):
         if self.h:

@@ -1,0 +1,85 @@
+"""The level predictor's HIP kernels against the CPU restatement (oracle/mlp.py), its training, and
+what the trained network gains inside a batched plan over the plain method."""
+import numpy as np
+import pytest
+
+from libgrpp_amd import configs
+from oracle import mlp as M
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def L():
+    from libgrpp_amd import learned
+    return learned
+
+
+def _random_model(L, seed):
+    rng = np.random.default_rng(seed)
+    net = L.MLP(seed)
+    p = rng.normal(0.0, 0.15, M.NPARAM).astype(np.float32)
+    net.params(set_to=p)
+    mean, scale = rng.normal(0.0, 0.5, M.NIN).astype(np.float32), rng.uniform(0.5, 1.5, M.NIN).astype(np.float32)
+    net.set_normalisation(mean, scale)
+    return net, p.astype(np.float64), mean.astype(np.float64), scale.astype(np.float64), rng
+
+
+@pytest.mark.parametrize("n", [1, 31, 100, 257])
+def test_gradient_matches_cpu(L, n):
+    """backward kernels + tile reduction vs the float64 restatement (FP32 tolerance)"""
+    net, p, mean, scale, rng = _random_model(L, n)
+    X = rng.normal(0.0, 2.0, (n, M.NIN)).astype(np.float32)
+    T = rng.normal(0.0, 1.0, (n, M.NOUT)).astype(np.float32)
+    g, loss = net.gradient(X, T)
+    gc = M.gradient(p, X.astype(np.float64), T.astype(np.float64), mean, scale)
+    lc = M.loss(p, X.astype(np.float64), T.astype(np.float64), mean, scale)
+    assert abs(loss - lc) <= 1e-5 * max(1.0, abs(lc))
+    assert np.abs(g - gc).max() <= 1e-4 * np.abs(gc).max() + 1e-7
+    net.close()
+
+
+def test_inference_fp32_and_bf16(L):
+    net, p, mean, scale, rng = _random_model(L, 5)
+    X = rng.normal(0.0, 2.0, (500, M.NIN)).astype(np.float32)
+    yc, _ = M.forward(p, X.astype(np.float64), mean, scale)
+    y32 = net.predict(X, bf16=False)
+    y16 = net.predict(X, bf16=True)
+    assert np.abs(y32 - yc).max() <= 1e-4 * np.abs(yc).max()
+    # BF16 weights (8-bit mantissa): a few percent of the output scale
+    assert np.abs(y16 - yc).max() <= 5e-2 * np.abs(yc).max()
+    net.close()
+
+
+def test_training_fits_a_smooth_target(L):
+    rng = np.random.default_rng(3)
+    X = rng.normal(0.0, 1.0, (4096, M.NIN)).astype(np.float32)
+    T = np.stack([np.tanh(X[:, 0] + 0.5 * X[:, 1]), X[:, 2] * X[:, 3], np.sin(X[:, 4])], axis=1).astype(np.float32)
+    net, loss = L.train_predictor(X, T, epochs=1, batch=256, lr=2e-3, seed=1)
+    first = loss
+    loss = net.train(X, T, epochs=30, batch=256, lr=2e-3, seed=2)
+    assert loss < 0.2 * first, (first, loss)
+    net.close()
+
+
+def test_learned_order_gains_and_keeps_results(L):
+    """trained on heavy_cluster seeds 1-2, applied to seed 0: fewer radial grid points than the plain
+    method, the same matrices (the order of rows only changes rounding)"""
+    from libgrpp_amd import lib as G
+    X, T = L.training_set(L.default_training_systems(2))
+    net, loss = L.train_predictor(X, T, epochs=30)
+    sysm = configs.heavy_cluster()
+    plan = G.Plan(sysm)
+    plan.execute()
+    plain = plan.download()
+    pts_plain = plan.stats()["n_radial_points"]
+    L.apply(plan, net)
+    plan.execute()
+    learned = plan.download()
+    pts_learned = plan.stats()["n_radial_points"]
+    plan.close()
+    net.close()
+    print(f"radial grid points: plain {pts_plain}, learned order {pts_learned}; training loss {loss:.3f}")
+    assert pts_learned < pts_plain
+    for a, b in zip(plain, learned):
+        assert np.abs(a - b).max() <= 1e-6 * max(1.0, np.abs(a).max())

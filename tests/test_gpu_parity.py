@@ -1,5 +1,5 @@
 """HIP engine (through the C ABI) against the oracle, the genuine libgrpp vectors and the
-reference-held goldens.  Tolerance: the BASELINE rule, |d| <= 1e-10 or |d|/|x| <= 1e-9 for |x| > 1
+reference-held goldens.  Tolerance: the parity rule, |d| <= 1e-10 or |d|/|x| <= 1e-9 for |x| > 1
 (libgrpp_amd.io.parity_error)."""
 import os
 
