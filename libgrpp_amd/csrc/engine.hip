@@ -1308,14 +1308,24 @@ __device__ __forceinline__ double int_gauss(int n, double a)
 
 constexpr int TS_MAX = 6144;   // upper bound of the LDS doubles for the (m, i, k2) intermediate
 
-__global__ __launch_bounds__(256) void k_contract(const WorkItem *__restrict__ items, DevJob J, DevTables T,
-                                                  double *__restrict__ out, int parts, int nm_max, int qs_len,
-                                                  int ts_len)
+// One workgroup per (shell pair, centre): the centre's contribution to the four blocks goes to its
+// own slice of the partial buffer; k_contract_reduce adds the slices of a pair in centre order.
+struct PartItem {
+    int item, ci;
+    long long off;   // first double of this part's slice: 4 operators x ncart(A) x ncart(B)
+};
+
+__global__ __launch_bounds__(256) void k_contract(const PartItem *__restrict__ partlist, const WorkItem *__restrict__ items,
+                                                  DevJob J, DevTables T, double *__restrict__ partial, int parts,
+                                                  int nm_max, int qs_len, int ts_len)
 {
     // dynamic LDS sized by the plan: Q block (qs_len doubles) + (m, i, k2) intermediate (ts_len)
     extern __shared__ double ct_lds[];
     double *Qs = ct_lds, *Ts = ct_lds + qs_len;
-    const WorkItem it = items[blockIdx.x];
+    const PartItem pt = partlist[blockIdx.x];
+    WorkItem it = items[pt.item];
+    it.cb = pt.ci;
+    it.ce = pt.ci + 1;
     const DShell sa = J.sh[it.a], sb = J.sh[it.b];
     const int la = sa.L, lb = sb.L, ncA = dev_ncart(la), ncB = dev_ncart(lb);
     const int tid = threadIdx.x;
@@ -1480,20 +1490,35 @@ __global__ __launch_bounds__(256) void k_contract(const WorkItem *__restrict__ i
                 }
             }
         }
-        // ---- accumulate into the output blocks written by k_type1 ----
+        // ---- this centre's slice of the partial blocks ----
 #pragma unroll
         for (int q = 0; q < CT_EPT; q++) {
             const int e = tid + q * 256;
             if (e >= nel) break;
             const int i = r0 + e / ncB, j = e % ncB;
 #pragma unroll
-            for (int op = 0; op < 4; op++) {
-                out[op * J.op_stride + it.off + (long long) i * it.stride + j] += acc[op][q];
-                if (it.mirror)
-                    out[op * J.op_stride + it.moff + (long long) j * it.stride + i] += (op == 0 ? 1.0 : -1.0) * acc[op][q];
-            }
+            for (int op = 0; op < 4; op++) partial[pt.off + ((long long) op * ncA + i) * ncB + j] = acc[op][q];
         }
         __syncthreads();
+    }
+}
+
+// out (written by k_type1) += sum over the item's centres, in centre order, of the partial slices;
+// the (B, A) block of a mirrored item is the (anti)symmetric image
+__global__ __launch_bounds__(256) void k_contract_reduce(const WorkItem *__restrict__ items,
+                                                         const int *__restrict__ item_part0,
+                                                         const PartItem *__restrict__ partlist, DevJob J,
+                                                         const double *__restrict__ partial, double *__restrict__ out)
+{
+    const WorkItem it = items[blockIdx.x];
+    const int ncA = dev_ncart(J.sh[it.a].L), ncB = dev_ncart(J.sh[it.b].L), nel = ncA * ncB;
+    const int p0 = item_part0[blockIdx.x], np = it.ce - it.cb;
+    for (int e = threadIdx.x; e < 4 * nel; e += blockDim.x) {
+        const int op = e / nel, ij = e % nel, i = ij / ncB, j = ij % ncB;
+        double s = 0.0;
+        for (int k = 0; k < np; k++) s += partial[partlist[p0 + k].off + e];
+        out[op * J.op_stride + it.off + (long long) i * it.stride + j] += s;
+        if (it.mirror) out[op * J.op_stride + it.moff + (long long) j * it.stride + i] += (op == 0 ? 1.0 : -1.0) * s;
     }
 }
 
@@ -1621,6 +1646,10 @@ struct eng_plan {
     int n_delta_centers = 0;
     WorkItem *d_items = nullptr;
     int n_items = 0;
+    PartItem *d_parts = nullptr;     // (item, centre) workgroups of the contraction
+    int *d_item_part0 = nullptr;     // first part of each item
+    double *d_partial = nullptr;     // per-part partial blocks
+    int n_parts = 0;
     unsigned long long *d_counters = nullptr;
     hipEvent_t ev[8] = {};   // execution phases: see eng_plan_execute
     // statistics
@@ -1968,6 +1997,17 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
         P->n_integrals += 4LL * ncart(sh[e.a].L) * ncart(sh[e.b].L) * ntask;
     }
     std::vector<int> cidx(center_index, center_index + nidx);
+    std::vector<PartItem> partlist;
+    std::vector<int> item_part0(std::max(1, nitem), 0);
+    long long partial_len = 0;
+    for (int i = 0; i < nitem; i++) {
+        item_part0[i] = (int) partlist.size();
+        const long long blk = 4LL * ncart(sh[items[i].a].L) * ncart(sh[items[i].b].L);
+        for (int ci = items[i].cb; ci < items[i].ce; ci++) {
+            partlist.push_back({i, ci, partial_len});
+            partial_len += blk;
+        }
+    }
     {
         // LDS of k_type1: E tables of a batch of primitive pairs (r^0 terms), or E plus two R/G
         // layers of one pair (r^-1, r^-2 terms); the block accumulator
@@ -2027,6 +2067,9 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     rc |= dput(P, &P->d_ocvi, ocv);
     rc |= dput(P, &P->d_delta_centers, delta_centers);
     rc |= dput(P, &P->d_items, witems);
+    rc |= dput(P, &P->d_parts, partlist);
+    rc |= dput(P, &P->d_item_part0, item_part0);
+    rc |= dalloc(P, &P->d_partial, (size_t) partial_len);
     rc |= dalloc(P, &P->d_F, (size_t) frows * NPTS);
     rc |= dalloc(P, &P->d_wU, (size_t) wpots.size() * NPTS);
     rc |= dalloc(P, &P->d_Q, (size_t) qlen);
@@ -2052,6 +2095,7 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     P->n_ocv = (int) ocv.size();
     P->n_delta_centers = (int) delta_centers.size();
     P->n_items = nitem;
+    P->n_parts = (int) partlist.size();
     P->nm_max = 2 * Lmax_pot + 1;
     P->rows_logical = rows;
     P->slot_rows = slot_rows;
@@ -2197,9 +2241,13 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
                            st, P->d_items, P->J, P->T, P->d_out, P->parts, P->t1_scratch);
     }
     HIPCHK(hipEventRecord(P->ev[6], st));
-    if (P->n_items > 0)
-        hipLaunchKernelGGL(k_contract, dim3(P->n_items), dim3(256), (size_t) (P->ct_qs + P->ct_ts) * sizeof(double),
-                           st, P->d_items, P->J, P->T, P->d_out, P->parts, P->nm_max, P->ct_qs, P->ct_ts);
+    if (P->n_parts > 0) {
+        hipLaunchKernelGGL(k_contract, dim3(P->n_parts), dim3(256), (size_t) (P->ct_qs + P->ct_ts) * sizeof(double),
+                           st, P->d_parts, P->d_items, P->J, P->T, P->d_partial, P->parts, P->nm_max, P->ct_qs,
+                           P->ct_ts);
+        hipLaunchKernelGGL(k_contract_reduce, dim3(P->n_items), dim3(256), 0, st, P->d_items, P->d_item_part0,
+                           P->d_parts, P->J, P->d_partial, P->d_out);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(P->ev[7], st));
     HIPCHK(hipEventSynchronize(P->ev[7]));
