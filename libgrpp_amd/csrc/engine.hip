@@ -120,6 +120,11 @@ struct OcvItem {
 struct OcEntry {
     int pot, shell;         // potential index, OC-shell index (in the shell array)
 };
+// one (shell pair, centre) of a work item and its slice of the partial buffer
+struct PartItem {
+    int item, ci;
+    long long off;   // first double of the slice: 4 operators x ncart(A) x ncart(B)
+};
 struct WorkItem {
     int a, b, cb, ce;
     long long off, moff;
@@ -1153,37 +1158,31 @@ constexpr int MMD_SCRATCH_BATCH = 4096;   // doubles for a batch of primitive-pa
 constexpr int MAX_BLOCK = 66 * 66;
 constexpr int MAX_OCB = 66 * 28;   // basis shell x outercore shell (L <= 6)
 
-__global__ __launch_bounds__(256) void k_type1(const WorkItem *__restrict__ items, DevJob J, DevTables T,
-                                               double *__restrict__ out, int parts, int scratch_len)
+// type-1 (local) contribution of one (shell pair, centre): it opens the part's slice of the partial
+// buffer (arep = type-1, spin-orbit = 0); k_contract adds the semilocal, SO and outercore terms
+__global__ __launch_bounds__(256) void k_type1(const PartItem *__restrict__ partlist, const WorkItem *__restrict__ items,
+                                               DevJob J, DevTables T, double *__restrict__ partial, int parts,
+                                               int scratch_len)
 {
     // dynamic LDS sized by the plan: E / R tables (scratch_len doubles), then the block accumulator
     extern __shared__ double t1_lds[];
     double *scratch = t1_lds, *acc = t1_lds + scratch_len;
-    const WorkItem it = items[blockIdx.x];
+    const PartItem pt = partlist[blockIdx.x];
+    const WorkItem it = items[pt.item];
     const DShell sa = J.sh[it.a], sb = J.sh[it.b];
     const int ncA = dev_ncart(sa.L), ncB = dev_ncart(sb.L), nel = ncA * ncB;
     for (int e = threadIdx.x; e < nel; e += blockDim.x) acc[e] = 0.0;
     __syncthreads();
-    if (parts & 1) {
+    const DCenter c = J.ctr[J.cidx[pt.ci]];
+    if ((parts & 1) && c.local >= 0) {
         const MmdShell A = {sa.L, sa.nprim, sa.poff, sa.x, sa.y, sa.z};
         const MmdShell B = {sb.L, sb.nprim, sb.poff, sb.x, sb.y, sb.z};
-        for (int ci = it.cb; ci < it.ce; ci++) {
-            const DCenter c = J.ctr[J.cidx[ci]];
-            if (c.local < 0) continue;
-            const DPot p = J.pot[c.local];
-            mmd_accumulate(A, B, c.x, c.y, c.z, J.tpow, J.tcoef, J.tzeta, p.toff, p.toff + p.nterm, J.alpha, J.craw,
-                           acc, scratch, scratch_len, T.boys);
-        }
+        const DPot p = J.pot[c.local];
+        mmd_accumulate(A, B, c.x, c.y, c.z, J.tpow, J.tcoef, J.tzeta, p.toff, p.toff + p.nterm, J.alpha, J.craw, acc,
+                       scratch, scratch_len, T.boys);
     }
-    for (int e = threadIdx.x; e < nel; e += blockDim.x) {
-        const int i = e / ncB, j = e % ncB;
-        out[it.off + (long long) i * it.stride + j] = acc[e];
-        if (it.mirror) out[it.moff + (long long) j * it.stride + i] = acc[e];
-        for (int op = 1; op < 4; op++) {
-            out[op * J.op_stride + it.off + (long long) i * it.stride + j] = 0.0;
-            if (it.mirror) out[op * J.op_stride + it.moff + (long long) j * it.stride + i] = 0.0;
-        }
-    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 4 * nel; e += blockDim.x) partial[pt.off + e] = (e < nel) ? acc[e] : 0.0;
 }
 
 // outercore vectors: [S (ncA x (2L+1)), U (ncA x (2L+1))] per (centre, shell, OC entry)
@@ -1308,13 +1307,9 @@ __device__ __forceinline__ double int_gauss(int n, double a)
 
 constexpr int TS_MAX = 6144;   // upper bound of the LDS doubles for the (m, i, k2) intermediate
 
-// One workgroup per (shell pair, centre): the centre's contribution to the four blocks goes to its
-// own slice of the partial buffer; k_contract_reduce adds the slices of a pair in centre order.
-struct PartItem {
-    int item, ci;
-    long long off;   // first double of this part's slice: 4 operators x ncart(A) x ncart(B)
-};
-
+// One workgroup per (shell pair, centre) (PartItem): the centre's contribution to the four blocks
+// goes to its own slice of the partial buffer; k_contract_reduce adds the slices of a pair in
+// centre order.
 __global__ __launch_bounds__(256) void k_contract(const PartItem *__restrict__ partlist, const WorkItem *__restrict__ items,
                                                   DevJob J, DevTables T, double *__restrict__ partial, int parts,
                                                   int nm_max, int qs_len, int ts_len)
@@ -1497,13 +1492,13 @@ __global__ __launch_bounds__(256) void k_contract(const PartItem *__restrict__ p
             if (e >= nel) break;
             const int i = r0 + e / ncB, j = e % ncB;
 #pragma unroll
-            for (int op = 0; op < 4; op++) partial[pt.off + ((long long) op * ncA + i) * ncB + j] = acc[op][q];
+            for (int op = 0; op < 4; op++) partial[pt.off + ((long long) op * ncA + i) * ncB + j] += acc[op][q];
         }
         __syncthreads();
     }
 }
 
-// out (written by k_type1) += sum over the item's centres, in centre order, of the partial slices;
+// out = sum over the item's centres, in centre order, of the partial slices;
 // the (B, A) block of a mirrored item is the (anti)symmetric image
 __global__ __launch_bounds__(256) void k_contract_reduce(const WorkItem *__restrict__ items,
                                                          const int *__restrict__ item_part0,
@@ -1517,8 +1512,8 @@ __global__ __launch_bounds__(256) void k_contract_reduce(const WorkItem *__restr
         const int op = e / nel, ij = e % nel, i = ij / ncB, j = ij % ncB;
         double s = 0.0;
         for (int k = 0; k < np; k++) s += partial[partlist[p0 + k].off + e];
-        out[op * J.op_stride + it.off + (long long) i * it.stride + j] += s;
-        if (it.mirror) out[op * J.op_stride + it.moff + (long long) j * it.stride + i] += (op == 0 ? 1.0 : -1.0) * s;
+        out[op * J.op_stride + it.off + (long long) i * it.stride + j] = s;
+        if (it.mirror) out[op * J.op_stride + it.moff + (long long) j * it.stride + i] = (op == 0 ? 1.0 : -1.0) * s;
     }
 }
 
@@ -2237,17 +2232,19 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
                                    (P->t1_scratch + P->t1_acc) * (int) sizeof(double)));
         HIPCHK(hipFuncSetAttribute((const void *) k_contract, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (P->ct_qs + P->ct_ts) * (int) sizeof(double)));
-        hipLaunchKernelGGL(k_type1, dim3(P->n_items), dim3(256), (size_t) (P->t1_scratch + P->t1_acc) * sizeof(double),
-                           st, P->d_items, P->J, P->T, P->d_out, P->parts, P->t1_scratch);
+        if (P->n_parts > 0)
+            hipLaunchKernelGGL(k_type1, dim3(P->n_parts), dim3(256),
+                               (size_t) (P->t1_scratch + P->t1_acc) * sizeof(double), st, P->d_parts, P->d_items, P->J,
+                               P->T, P->d_partial, P->parts, P->t1_scratch);
     }
     HIPCHK(hipEventRecord(P->ev[6], st));
-    if (P->n_parts > 0) {
+    if (P->n_parts > 0)
         hipLaunchKernelGGL(k_contract, dim3(P->n_parts), dim3(256), (size_t) (P->ct_qs + P->ct_ts) * sizeof(double),
                            st, P->d_parts, P->d_items, P->J, P->T, P->d_partial, P->parts, P->nm_max, P->ct_qs,
                            P->ct_ts);
+    if (P->n_items > 0)   // also writes the zero blocks of items without centres
         hipLaunchKernelGGL(k_contract_reduce, dim3(P->n_items), dim3(256), 0, st, P->d_items, P->d_item_part0,
                            P->d_parts, P->J, P->d_partial, P->d_out);
-    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(P->ev[7], st));
     HIPCHK(hipEventSynchronize(P->ev[7]));
