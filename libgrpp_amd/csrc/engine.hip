@@ -399,13 +399,14 @@ __device__ __forceinline__ double ipow(double x, int n)
 // after four evaluations fails the whole estimate (the reference's five-step rule).  The estimate
 // only steers the quadrature (screened-out threshold, forced refinement), so few-ulp differences
 // of these algebraically equal forms are immaterial.
-__device__ __forceinline__ bool screen_triple(const DevTables &T, int n, int l1, int l2, double aA, double aB,
-                                              double eta, double CA, double CB, double k1, double k2, double &val,
-                                              unsigned &nstep)
+__device__ __forceinline__ bool screen_fixed_point(const DevTables &T, int n, int l1, int l2, double aA, double aB,
+                                                   double eta, double CA, double CB, double k1, double k2, double &r0,
+                                                   unsigned &nstep)
 {
     const double p = aA + aB + eta, invp = 1.0 / p, inv2p = 0.5 * invp;
     const double nsh = (double) (n - (l1 > 0 ? l1 + 1 : 0) - (l2 > 0 ? l2 + 1 : 0));
-    double r0 = (aA * CA + aB * CB) * invp, r0p;
+    double r0p;
+    r0 = (aA * CA + aB * CB) * invp;
     int steps = 0;
     do {
         if (++steps == 5) return false;
@@ -415,10 +416,15 @@ __device__ __forceinline__ bool screen_triple(const DevTables &T, int n, int l1,
         const double a = bessel_ratio(T, l1, k1r) * k1r + bessel_ratio(T, l2, k2r) * k2r + nsh;
         r0 = sqrt(a * inv2p);
     } while (fabs(r0 - r0p) > 1e-2);
-    const double dA = r0 - CA, dB = r0 - CB;
-    val = sqrt(M_PI * invp) * ipow(r0, n) * bessel_k_steer(T, l1, k1 * r0) * bessel_k_steer(T, l2, k2 * r0) *
-          exp(-eta * r0 * r0 - aA * dA * dA - aB * dB * dB) * 0.5 * (1.0 + erf(sqrt(p) * r0));
     return true;
+}
+
+__device__ __forceinline__ double screen_value(const DevTables &T, int n, int l1, int l2, double aA, double aB,
+                                              double eta, double CA, double CB, double k1, double k2, double r0)
+{
+    const double p = aA + aB + eta, dA = r0 - CA, dB = r0 - CB;
+    return sqrt(M_PI / p) * ipow(r0, n) * bessel_k_steer(T, l1, k1 * r0) * bessel_k_steer(T, l2, k2 * r0) *
+           exp(-eta * r0 * r0 - aA * dA * dA - aB * dB * dB) * 0.5 * (1.0 + erf(sqrt(p) * r0));
 }
 
 // The envelope of one radial entry (screening.c:193-248), evaluated by a whole wave: lane q takes
@@ -435,27 +441,37 @@ __device__ double screen_entry_wave(const DevJob &J, const DevTables &T, const D
     double est = 0.0;
 #pragma unroll 1
     for (int q0 = 0; q0 < ntot; q0 += 64) {
+        // fixed points of the round's 64 triples first: a failure ends the entry before any value
+        // is computed (most entries of a typical workload fail)
         const int q = q0 + lane;
-        bool fail = false;
+        bool active = false, analytic = false, fail = false;
+        double ci = 0.0, cj = 0.0, aA = 0.0, aB = 0.0, k1 = 0.0, k2 = 0.0, eta = 0.0, coef = 0.0, r0 = 0.0;
+        int n = 0;
         if (q < ntot) {
             const int i = q / njt, r = q - i * njt, j = r / nt, t = r - j * nt;
-            const double ci = J.cnorm[A.poff + i], cj = J.cnorm[B.poff + j];
-            const double aA = J.alpha[A.poff + i], k1 = 2 * aA * CA;
-            const double aB = J.alpha[B.poff + j], k2 = 2 * aB * CB;
+            ci = J.cnorm[A.poff + i];
+            cj = J.cnorm[B.poff + j];
+            aA = J.alpha[A.poff + i];
+            aB = J.alpha[B.poff + j];
+            k1 = 2 * aA * CA;
+            k2 = 2 * aB * CB;
             // primitives dropped by libgrpp_shell_shrink; K_l(0) = 0 for l >= 1
-            if (ci != 0.0 && cj != 0.0 && !((l1 >= 1 && fabs(k1) <= 1e-14) || (l2 >= 1 && fabs(k2) <= 1e-14))) {
-                const double eta = J.tzeta[P.toff + t];
-                const int n = N + J.tpow[P.toff + t];
-                double val = 0.0;
+            active = ci != 0.0 && cj != 0.0 && !((l1 >= 1 && fabs(k1) <= 1e-14) || (l2 >= 1 && fabs(k2) <= 1e-14));
+            if (active) {
+                eta = J.tzeta[P.toff + t];
+                coef = J.tcoef[P.toff + t];
+                n = N + J.tpow[P.toff + t];
+                analytic = (l1 == 0 && l2 == 0 && fabs(k1) <= 1e-14 && fabs(k2) <= 1e-14);
                 ntrip++;
-                if (l1 == 0 && l2 == 0 && fabs(k1) <= 1e-14 && fabs(k2) <= 1e-14)
-                    val = exp(-aA * CA * CA - aB * CB * CB) * int_gauss_s(n, aA + aB + eta);
-                else
-                    fail = !screen_triple(T, n, l1, l2, aA, aB, eta, CA, CB, k1, k2, val, nstep);
-                if (!fail) est += ci * cj * J.tcoef[P.toff + t] * val;
+                if (!analytic) fail = !screen_fixed_point(T, n, l1, l2, aA, aB, eta, CA, CB, k1, k2, r0, nstep);
             }
         }
         if (__any(fail)) return __longlong_as_double(0x7ff8000000000000LL);
+        if (active) {
+            const double val = analytic ? exp(-aA * CA * CA - aB * CB * CB) * int_gauss_s(n, aA + aB + eta)
+                                        : screen_value(T, n, l1, l2, aA, aB, eta, CA, CB, k1, k2, r0);
+            est += ci * cj * coef * val;
+        }
     }
     for (int o = 32; o > 0; o >>= 1) est += __shfl_xor(est, o);
     return est;
