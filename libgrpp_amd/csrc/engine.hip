@@ -445,6 +445,7 @@ __device__ double screen_entry_wave(const DevJob &J, const DevTables &T, const D
         // is computed (most entries of a typical workload fail)
         const int q = q0 + lane;
         bool active = false, analytic = false, fail = false;
+        unsigned rsteps = 0;
         double ci = 0.0, cj = 0.0, aA = 0.0, aB = 0.0, k1 = 0.0, k2 = 0.0, eta = 0.0, coef = 0.0, r0 = 0.0;
         int n = 0;
         if (q < ntot) {
@@ -462,11 +463,18 @@ __device__ double screen_entry_wave(const DevJob &J, const DevTables &T, const D
                 coef = J.tcoef[P.toff + t];
                 n = N + J.tpow[P.toff + t];
                 analytic = (l1 == 0 && l2 == 0 && fabs(k1) <= 1e-14 && fabs(k2) <= 1e-14);
-                ntrip++;
-                if (!analytic) fail = !screen_fixed_point(T, n, l1, l2, aA, aB, eta, CA, CB, k1, k2, r0, nstep);
+                if (!analytic) fail = !screen_fixed_point(T, n, l1, l2, aA, aB, eta, CA, CB, k1, k2, r0, rsteps);
             }
         }
-        if (__any(fail)) return __longlong_as_double(0x7ff8000000000000LL);
+        // work accounting follows the reference's order: a failed entry counts the triples up to
+        // and including its first failing one
+        const unsigned long long fmask = __ballot(fail);
+        const int qfail = fmask ? q0 + __ffsll((long long) fmask) - 1 : ntot;
+        if (active && q <= qfail) {
+            ntrip++;
+            nstep += rsteps;
+        }
+        if (fmask) return __longlong_as_double(0x7ff8000000000000LL);
         if (active) {
             const double val = analytic ? exp(-aA * CA * CA - aB * CB * CB) * int_gauss_s(n, aA + aB + eta)
                                         : screen_value(T, n, l1, l2, aA, aB, eta, CA, CB, k1, k2, r0);
