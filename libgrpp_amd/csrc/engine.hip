@@ -143,6 +143,8 @@ struct DevJob {
     // lookup tables, slot = (centre, L) pair
     const int *slot_rowoff;        // [slot * nshell + A]
     const int *rowpos;             // [slot row base + logical row] -> physical row of the radial table
+    const int *pperm;              // [poff + i] -> primitive of the shell with the i-th smallest exponent
+    const int *tperm;              // [toff + t] -> potential term with the t-th smallest exponent
     const long long *slot_poff;    // [slot * nshell + A]
     const long long *slot_ploff;   // [slot * nshell + A] (-1 if the slot has no SO channel)
     const long long *slot_pb2;     // bra projector, type-2 thresholds
@@ -427,6 +429,24 @@ __device__ __forceinline__ double screen_value(const DevTables &T, int n, int l1
            exp(-eta * r0 * r0 - aA * dA * dA - aB * dB * dB) * 0.5 * (1.0 + erf(sqrt(p) * r0));
 }
 
+constexpr int SCR_PROBE = 8;   // triples per entry in the screening probe (eight entries per wave)
+
+// Fixed point of triple q of an entry in the probe order (smallest exponents first: the triples most
+// likely to fail); active is false for triples that vanish
+__device__ __forceinline__ bool screen_probe_triple(const DevJob &J, const DevTables &T, const DShell &A,
+                                                    const DShell &B, double CA, double CB, const DPot &P, int l1,
+                                                    int l2, int N, int q, bool &active, unsigned &rsteps)
+{
+    const int nt = P.nterm, njt = B.nprim * nt;
+    const int iq = q / njt, r = q - iq * njt, jq = r / nt, tq = r - jq * nt;
+    const int i = J.pperm[A.poff + iq], j = J.pperm[B.poff + jq], t = J.tperm[P.toff + tq];
+    const double aA = J.alpha[i], aB = J.alpha[j], k1 = 2 * aA * CA, k2 = 2 * aB * CB;
+    active = J.cnorm[i] != 0.0 && J.cnorm[j] != 0.0 && !((l1 >= 1 && fabs(k1) <= 1e-14) || (l2 >= 1 && fabs(k2) <= 1e-14));
+    if (!active || (l1 == 0 && l2 == 0 && fabs(k1) <= 1e-14 && fabs(k2) <= 1e-14)) return false;
+    double r0;
+    return !screen_fixed_point(T, N + J.tpow[t], l1, l2, aA, aB, J.tzeta[t], CA, CB, k1, k2, r0, rsteps);
+}
+
 // The envelope of one radial entry (screening.c:193-248), evaluated by a whole wave: lane q takes
 // the primitive triples q, q+64, ... of the (i, j, t) space, so the lanes of a wave share the entry's
 // (lambda1, lambda2, N) and run the same code.  A triple that fails the fixed point fails the
@@ -449,11 +469,13 @@ __device__ double screen_entry_wave(const DevJob &J, const DevTables &T, const D
         double ci = 0.0, cj = 0.0, aA = 0.0, aB = 0.0, k1 = 0.0, k2 = 0.0, eta = 0.0, coef = 0.0, r0 = 0.0;
         int n = 0;
         if (q < ntot) {
-            const int i = q / njt, r = q - i * njt, j = r / nt, t = r - j * nt;
-            ci = J.cnorm[A.poff + i];
-            cj = J.cnorm[B.poff + j];
-            aA = J.alpha[A.poff + i];
-            aB = J.alpha[B.poff + j];
+            // triples in the order of the screening probe (smallest exponents first)
+            const int iq = q / njt, r = q - iq * njt, jq = r / nt, tq = r - jq * nt;
+            const int i = J.pperm[A.poff + iq], j = J.pperm[B.poff + jq], t = J.tperm[P.toff + tq] - P.toff;
+            ci = J.cnorm[i];
+            cj = J.cnorm[j];
+            aA = J.alpha[i];
+            aB = J.alpha[j];
             k1 = 2 * aA * CA;
             k2 = 2 * aB * CB;
             // primitives dropped by libgrpp_shell_shrink; K_l(0) = 0 for l >= 1
@@ -494,6 +516,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     // full-cost ones first and the ones that vanish analytically (K_l(0) = 0 for l >= 1 on a shell
     // sitting on the centre) last, so lanes run entries of like cost side by side.
     __shared__ int list[RT * RT / 4];
+    __shared__ unsigned char settled[RT * RT / 4];   // entries whose probe found a failing triple
     __shared__ int n_full, n_zero;
     const TileItem t = tiles[blockIdx.x >> 2];
     const QJob jb = jobs[t.job];
@@ -532,12 +555,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
         scr[jb.qoff + (long long) row * jb.M + col] = 0.0;
         scr[jb.qoff + (long long) col * jb.M + row] = 0.0;
     }
-    // the others: the whole wave per entry
-#pragma unroll 1
-    for (int q = 0; q < nf; q++) {
+    // an entry with a triple that fails the fixed point has no estimate (NaN), whatever its other
+    // triples give; the triples most likely to fail come first in the probe order, so a probe of
+    // the first SCR_PROBE triples of eight entries at once settles most entries of a workload
+    auto entry = [&](int q, int &row, int &col, RowDesc &ra, RowDesc &rb, DShell &A, DShell &B, double &CA,
+                     double &CB) {
         const int e = list[q];
-        const int row = t.ti * RT + (e / RT) * 4 + sub, col = t.tj * RT + e % RT;
-        RowDesc ra = R[row], rb = R[col];
+        row = t.ti * RT + (e / RT) * 4 + sub;
+        col = t.tj * RT + e % RT;
+        ra = R[row];
+        rb = R[col];
         // a fixed orientation of the (A, B, lambda1, lambda2, N) tuple so that every split and both
         // triangles get bitwise the same estimate
         if (ra.shell > rb.shell || (ra.shell == rb.shell && ra.lam > rb.lam)) {
@@ -545,9 +572,51 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
             ra = rb;
             rb = tmp;
         }
-        const DShell A = J.sh[ra.shell], B = J.sh[rb.shell];
+        A = J.sh[ra.shell];
+        B = J.sh[rb.shell];
         const double ax = c.x - A.x, ay = c.y - A.y, az = c.z - A.z, bx = c.x - B.x, by = c.y - B.y, bz = c.z - B.z;
-        const double CA = sqrt(ax * ax + ay * ay + az * az), CB = sqrt(bx * bx + by * by + bz * bz);
+        CA = sqrt(ax * ax + ay * ay + az * az);
+        CB = sqrt(bx * bx + by * by + bz * bz);
+    };
+    const double NaN = __longlong_as_double(0x7ff8000000000000LL);
+#pragma unroll 1
+    for (int g0 = 0; g0 < nf; g0 += 64 / SCR_PROBE) {
+        const int g = threadIdx.x / SCR_PROBE, k = threadIdx.x % SCR_PROBE, qi = g0 + g;
+        bool fail = false, active = false;
+        unsigned rsteps = 0;
+        int row = 0, col = 0;
+        if (qi < nf) {
+            RowDesc ra, rb;
+            DShell A, B;
+            double CA, CB;
+            entry(qi, row, col, ra, rb, A, B, CA, CB);
+            if (k < A.nprim * B.nprim * P.nterm)
+                fail = screen_probe_triple(J, T, A, B, CA, CB, P, ra.lam, rb.lam, ra.n + rb.n, k, active, rsteps);
+        }
+        const unsigned long long fmask = __ballot(fail);
+        const unsigned gbits = (unsigned) (fmask >> (SCR_PROBE * g)) & ((1u << SCR_PROBE) - 1u);
+        if (gbits) {
+            if (active && k <= __ffs(gbits) - 1) {
+                ntrip++;
+                nstep += rsteps;
+            }
+            if (k == 0) {
+                scr[jb.qoff + (long long) row * jb.M + col] = NaN;
+                scr[jb.qoff + (long long) col * jb.M + row] = NaN;
+            }
+        }
+        if (k == 0 && qi < nf) settled[qi] = gbits ? 1 : 0;
+    }
+    __syncthreads();
+    // the entries the probe did not settle: the whole wave per entry
+#pragma unroll 1
+    for (int q = 0; q < nf; q++) {
+        if (settled[q]) continue;
+        int row, col;
+        RowDesc ra, rb;
+        DShell A, B;
+        double CA, CB;
+        entry(q, row, col, ra, rb, A, B, CA, CB);
         const double v = screen_entry_wave(J, T, A, B, CA, CB, P, ra.lam, rb.lam, ra.n + rb.n, ntrip, nstep);
         if (threadIdx.x == 0) {
             scr[jb.qoff + (long long) row * jb.M + col] = v;
@@ -2056,12 +2125,30 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     double *d_alpha, *d_craw, *d_cnorm, *d_tcoef, *d_tzeta;
     DCenter *d_ctr;
     DPot *d_pot;
-    int *d_tpow, *d_cidx, *d_slot_rowoff;
+    int *d_tpow, *d_cidx, *d_slot_rowoff, *d_pperm, *d_tperm;
     OcEntry *d_oce;
     long long *d_slot_poff, *d_slot_ploff, *d_ocv_tab, *d_slot_pb2, *d_slot_pb3;
     int rc = 0;
     rc |= dput(P, &d_sh, sh);
     rc |= dput(P, &d_alpha, alpha);
+    {
+        // screening probe order: each shell's primitives and each potential's terms by increasing
+        // exponent (measured on heavy-atom clusters: the diffuse triples are the ones whose
+        // envelope maximum does not settle, so they settle most failing entries in the probe)
+        std::vector<int> pperm(alpha.size()), tperm(tzeta.size());
+        for (const DShell &d : sh) {
+            for (int i = 0; i < d.nprim; i++) pperm[d.poff + i] = d.poff + i;
+            std::stable_sort(pperm.begin() + d.poff, pperm.begin() + d.poff + d.nprim,
+                             [&](int a, int b) { return alpha[a] < alpha[b]; });
+        }
+        for (const DPot &d : pots) {
+            for (int k = 0; k < d.nterm; k++) tperm[d.toff + k] = d.toff + k;
+            std::stable_sort(tperm.begin() + d.toff, tperm.begin() + d.toff + d.nterm,
+                             [&](int a, int b) { return tzeta[a] < tzeta[b]; });
+        }
+        rc |= dput(P, &d_pperm, pperm);
+        rc |= dput(P, &d_tperm, tperm);
+    }
     rc |= dput(P, &d_craw, craw);
     rc |= dput(P, &d_cnorm, cnorm);
     rc |= dput(P, &d_ctr, ctr);
@@ -2179,6 +2266,8 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     J.pot = d_pot;
     J.tpow = d_tpow;
     J.rowpos = P->d_rowpos;
+    J.pperm = d_pperm;
+    J.tperm = d_tperm;
     J.tcoef = d_tcoef;
     J.tzeta = d_tzeta;
     J.cidx = d_cidx;
