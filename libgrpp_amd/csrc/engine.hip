@@ -814,7 +814,7 @@ __global__ __launch_bounds__(256) void k_radial(const TileItem *__restrict__ til
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 template <bool REC>
-__global__ __launch_bounds__(256) void k_radial_mfma(const TileItem *__restrict__ tiles, const QJob *__restrict__ jobs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_radial_mfma(const TileItem *__restrict__ tiles, const QJob *__restrict__ jobs,
                                                      const RowDesc *__restrict__ rows, const double *__restrict__ F,
                                                      const double *__restrict__ rpow, const double *__restrict__ wU,
                                                      double *__restrict__ Q, int min_level, double tol,
@@ -1403,7 +1403,7 @@ constexpr int TS_MAX = 6144;   // upper bound of the LDS doubles for the (m, i, 
 // One workgroup per (shell pair, centre) (PartItem): the centre's contribution to the four blocks
 // goes to its own slice of the partial buffer; k_contract_reduce adds the slices of a pair in
 // centre order.
-__global__ __launch_bounds__(256) void k_contract(const PartItem *__restrict__ partlist, const WorkItem *__restrict__ items,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_contract(const PartItem *__restrict__ partlist, const WorkItem *__restrict__ items,
                                                   DevJob J, DevTables T, double *__restrict__ partial, int parts,
                                                   int nm_max, int qs_len, int ts_len)
 {
