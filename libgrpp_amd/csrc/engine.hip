@@ -530,11 +530,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     }
     __syncthreads();
     constexpr int NE = RT * RT / 4;
+    // the quarter tiles cover the table in its logical row order (shell, n, lambda), whatever the
+    // physical order of the rows, so the work of a wave does not depend on a row permutation
+    auto phys = [&](int l) { return J.rowpos[jb.rows + l]; };
     for (int e = threadIdx.x; e < NE; e += blockDim.x) {
-        const int row = t.ti * RT + (e / RT) * 4 + sub, col = t.tj * RT + e % RT;
-        if (row >= jb.M || col >= jb.M) continue;
-        if (t.ti == t.tj && row > col) continue;
-        const RowDesc ra = R[row], rb = R[col];
+        const int lrow = t.ti * RT + (e / RT) * 4 + sub, lcol = t.tj * RT + e % RT;
+        if (lrow >= jb.M || lcol >= jb.M) continue;
+        if (t.ti == t.tj && lrow > lcol) continue;
+        const RowDesc ra = R[phys(lrow)], rb = R[phys(lcol)];
         const DShell A = J.sh[ra.shell], B = J.sh[rb.shell];
         // the canonical split of (lambda1, lambda2, N) may sit on either side of the diagonal
         // (rows need not be in (shell, n) order), so both orientations are tested
@@ -551,7 +554,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     // analytically vanishing entries: every triple is zero, so the estimate is 0 (screened out)
     for (int q = threadIdx.x; q < nz; q += blockDim.x) {
         const int e = list[NE - nz + q];
-        const int row = t.ti * RT + (e / RT) * 4 + sub, col = t.tj * RT + e % RT;
+        const int row = phys(t.ti * RT + (e / RT) * 4 + sub), col = phys(t.tj * RT + e % RT);
         scr[jb.qoff + (long long) row * jb.M + col] = 0.0;
         scr[jb.qoff + (long long) col * jb.M + row] = 0.0;
     }
@@ -561,8 +564,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     auto entry = [&](int q, int &row, int &col, RowDesc &ra, RowDesc &rb, DShell &A, DShell &B, double &CA,
                      double &CB) {
         const int e = list[q];
-        row = t.ti * RT + (e / RT) * 4 + sub;
-        col = t.tj * RT + e % RT;
+        row = phys(t.ti * RT + (e / RT) * 4 + sub);
+        col = phys(t.tj * RT + e % RT);
         ra = R[row];
         rb = R[col];
         // a fixed orientation of the (A, B, lambda1, lambda2, N) tuple so that every split and both
