@@ -6,7 +6,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, golden_matrix
+from conftest import GOLDEN, golden_matrix, within_noise
 from libgrpp_amd import configs, io
 
 pytestmark = pytest.mark.gpu
@@ -34,6 +34,23 @@ def test_batched_matches_oracle_and_goldens(G, orc, case, request):
     for k, nm in enumerate(OPS):
         _check(M[k], R[k], f"{case} {nm} vs oracle")
         _check(np.tril(M[k]), golden_matrix(case, nm), f"{case} {nm} vs golden")
+
+
+def test_batched_kcs_matches_oracle_and_goldens(G, kcs, kcs_oracle):
+    """KCs (304 functions, l up to 4 on Cs): spin-orbit to the parity rule against the oracle and the
+    goldens; arep element-wise to the rule or within 10x the formulation's rounding spread (oracle
+    with vs without FMA; up to 7.2e-10 on this matrix)"""
+    M = G.batch_matrices(kcs)
+    R, N = kcs_oracle
+    for k, nm in enumerate(OPS):
+        spread = np.abs(R[k] - N[k]).max()
+        for want, what in [(R[k], "oracle"), (golden_matrix("KCs", nm), "golden")]:
+            got = np.tril(M[k]) if what == "golden" else M[k]
+            ok = within_noise(got, want, spread)
+            print(f"KCs {nm} vs {what}: max abs {np.abs(got - want).max():.3e}, spread {spread:.3e}", flush=True)
+            assert ok.all(), (nm, what, int((~ok).sum()))
+            if k > 0:
+                assert io.parity_error(got, want)[2], (nm, what)
 
 
 def test_batched_small_pair_matches_reference_vectors(G):
