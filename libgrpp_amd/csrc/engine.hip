@@ -1742,7 +1742,9 @@ struct eng_plan {
     double *d_partial = nullptr;     // per-part partial blocks
     int n_parts = 0;
     unsigned long long *d_counters = nullptr;
-    hipEvent_t ev[8] = {};   // execution phases: see eng_plan_execute
+    hipEvent_t ev[8] = {};   // execution phases on the caller's stream: see eng_plan_execute
+    hipStream_t side = nullptr;   // second stream (default priority) for the radial-independent kernels
+    hipEvent_t evs[3] = {};       // phases on the side stream
     // statistics
     long long n_tasks = 0, n_integrals = 0, n_entries = 0;
     double flops_fixed = 0.0;    // contraction + type-1 + projector work (grid independent)
@@ -2288,6 +2290,8 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     J.nshell = nshell;   // lookup tables are indexed by basis shell; outercore shells follow them in sh
     J.op_stride = op_stride;
     for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&P->ev[i]));
+    for (int i = 0; i < 3; i++) HIPCHK(hipEventCreate(&P->evs[i]));
+    HIPCHK(hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking));
     {
         // radial quadrature on the matrix cores unless GRPP_RADIAL_KERNEL=valu
         const char *k = getenv("GRPP_RADIAL_KERNEL");
@@ -2300,10 +2304,34 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
 extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
 {
     hipStream_t st = (hipStream_t) stream_;
-    // events: 0 start | tables | 1 | screening | 2 | radial product | 3 | canonical splits | 4 |
-    // projectors, outercore vectors | 5 | type-1 | 6 | contraction | 7
+    // Two streams of default priority.  The caller's stream runs the radial chain (tables,
+    // screening, radial product, canonical splits); the side stream runs what does not depend on
+    // it (projectors, outercore vectors, type-1), forked after the counters are cleared and joined
+    // before the contraction.  Events: caller 0 start | tables | 1 | screening | 2 | radial | 3 |
+    // canonical | 4 ... join ... 6 | contraction | 7; side s0 | projectors, outercore | s1 | type-1 | s2.
     HIPCHK(hipMemsetAsync(P->d_counters, 0, 8 * sizeof(unsigned long long), st));
     HIPCHK(hipEventRecord(P->ev[0], st));
+    HIPCHK(hipStreamWaitEvent(P->side, P->ev[0], 0));
+    hipStream_t sd = P->side;
+    HIPCHK(hipEventRecord(P->evs[0], sd));
+    if (P->n_proj > 0) hipLaunchKernelGGL(k_proj, dim3(P->n_proj), dim3(256), 0, sd, P->d_proj, P->J, P->T, P->d_P);
+    if (P->n_delta_centers > 0)
+        hipLaunchKernelGGL(k_delta, dim3(P->n_delta_centers), dim3(64), 0, sd, P->d_delta_centers,
+                           P->n_delta_centers, P->J, P->d_delta);
+    if (P->n_ocv > 0) hipLaunchKernelGGL(k_ocv, dim3(P->n_ocv), dim3(256), 0, sd, P->d_ocvi, P->J, P->T, P->d_OCV);
+    HIPCHK(hipEventRecord(P->evs[1], sd));
+    if (P->n_items > 0) {
+        // dynamic LDS beyond the 64 KiB default (up to the 160 KiB of a gfx950 CU) must be opted into
+        HIPCHK(hipFuncSetAttribute((const void *) k_type1, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (P->t1_scratch + P->t1_acc) * (int) sizeof(double)));
+        HIPCHK(hipFuncSetAttribute((const void *) k_contract, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (P->ct_qs + P->ct_ts) * (int) sizeof(double)));
+        if (P->n_parts > 0)
+            hipLaunchKernelGGL(k_type1, dim3(P->n_parts), dim3(256),
+                               (size_t) (P->t1_scratch + P->t1_acc) * sizeof(double), sd, P->d_parts, P->d_items, P->J,
+                               P->T, P->d_partial, P->parts, P->t1_scratch);
+    }
+    HIPCHK(hipEventRecord(P->evs[2], sd));
     if (P->n_tab > 0) {
         dim3 g(NPTS / 256, P->n_tab);
         hipLaunchKernelGGL(k_tab_F, g, dim3(256), 0, st, P->d_tab, P->J, P->T, P->d_F);
@@ -2336,23 +2364,7 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
         hipLaunchKernelGGL(k_canon, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->J, P->d_Q,
                            P->d_Qc);
     HIPCHK(hipEventRecord(P->ev[4], st));
-    if (P->n_proj > 0) hipLaunchKernelGGL(k_proj, dim3(P->n_proj), dim3(256), 0, st, P->d_proj, P->J, P->T, P->d_P);
-    if (P->n_delta_centers > 0)
-        hipLaunchKernelGGL(k_delta, dim3(P->n_delta_centers), dim3(64), 0, st, P->d_delta_centers,
-                           P->n_delta_centers, P->J, P->d_delta);
-    if (P->n_ocv > 0) hipLaunchKernelGGL(k_ocv, dim3(P->n_ocv), dim3(256), 0, st, P->d_ocvi, P->J, P->T, P->d_OCV);
-    HIPCHK(hipEventRecord(P->ev[5], st));
-    if (P->n_items > 0) {
-        // dynamic LDS beyond the 64 KiB default (up to the 160 KiB of a gfx950 CU) must be opted into
-        HIPCHK(hipFuncSetAttribute((const void *) k_type1, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (P->t1_scratch + P->t1_acc) * (int) sizeof(double)));
-        HIPCHK(hipFuncSetAttribute((const void *) k_contract, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (P->ct_qs + P->ct_ts) * (int) sizeof(double)));
-        if (P->n_parts > 0)
-            hipLaunchKernelGGL(k_type1, dim3(P->n_parts), dim3(256),
-                               (size_t) (P->t1_scratch + P->t1_acc) * sizeof(double), st, P->d_parts, P->d_items, P->J,
-                               P->T, P->d_partial, P->parts, P->t1_scratch);
-    }
+    HIPCHK(hipStreamWaitEvent(st, P->evs[2], 0));   // join: the contraction reads projectors and type-1 slices
     HIPCHK(hipEventRecord(P->ev[6], st));
     if (P->n_parts > 0)
         hipLaunchKernelGGL(k_contract, dim3(P->n_parts), dim3(256), (size_t) (P->ct_qs + P->ct_ts) * sizeof(double),
@@ -2364,9 +2376,11 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(P->ev[7], st));
     HIPCHK(hipEventSynchronize(P->ev[7]));
+    const hipEvent_t *pb[7] = {&P->ev[0], &P->ev[1], &P->ev[2], &P->ev[3], &P->evs[0], &P->evs[1], &P->ev[6]};
+    const hipEvent_t *pe[7] = {&P->ev[1], &P->ev[2], &P->ev[3], &P->ev[4], &P->evs[1], &P->evs[2], &P->ev[7]};
     for (int i = 0; i < 7; i++) {
         float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, P->ev[i], P->ev[i + 1]));
+        HIPCHK(hipEventElapsedTime(&ms, *pb[i], *pe[i]));
         P->phase_seconds[i] = ms * 1e-3;
     }
     float ms = 0;
@@ -2397,6 +2411,9 @@ extern "C" void eng_plan_destroy(eng_plan_t *P)
     for (void *b : P->bufs) (void) hipFree(b);
     for (int i = 0; i < 8; i++)
         if (P->ev[i]) (void) hipEventDestroy(P->ev[i]);
+    for (int i = 0; i < 3; i++)
+        if (P->evs[i]) (void) hipEventDestroy(P->evs[i]);
+    if (P->side) (void) hipStreamDestroy(P->side);
     delete P;
 }
 
