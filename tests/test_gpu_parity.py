@@ -53,6 +53,21 @@ def test_batched_kcs_matches_oracle_and_goldens(G, kcs, kcs_oracle):
                 assert io.parity_error(got, want)[2], (nm, what)
 
 
+def test_batched_kcs_gradients_match_goldens(G, kcs):
+    """two KCs gradient goldens of the reference's test (d/dx and d/dy of the Cs atom): spin-orbit
+    to the parity rule; arep, whose elements reach 8e2, within 10x the formulation's rounding spread
+    on that matrix (oracle with vs without FMA: 6.2e-8, tests/golden/make_kcs_spread.py)"""
+    import json
+    spread = json.load(open(os.path.join(GOLDEN, "KCs", "gradient_spread.json")))
+    Gr, _ = G.batch_gradients(kcs)
+    for nm, (op, at, c) in {"arep_grad_1x": (0, 1, 0), "so_z_grad_1y": (3, 1, 1)}.items():
+        got, want = np.tril(Gr[op, 3 * at + c]), golden_matrix("KCs", nm)
+        print(f"KCs {nm}: max abs {np.abs(got - want).max():.3e}, spread {spread[nm]:.3e}", flush=True)
+        assert within_noise(got, want, spread[nm]).all(), nm
+        if op > 0:
+            assert io.parity_error(got, want)[2], nm
+
+
 def test_batched_small_pair_matches_reference_vectors(G):
     d = np.load(os.path.join(GOLDEN, "small_pair_reference.npz"))
     M = G.batch_matrices(configs.small_pair())
