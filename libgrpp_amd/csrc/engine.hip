@@ -2306,14 +2306,25 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
     hipStream_t st = (hipStream_t) stream_;
     // Two streams of default priority.  The caller's stream runs the radial chain (tables,
     // screening, radial product, canonical splits); the side stream runs what does not depend on
-    // it (projectors, outercore vectors, type-1), forked after the counters are cleared and joined
-    // before the contraction.  Events: caller 0 start | tables | 1 | screening | 2 | radial | 3 |
-    // canonical | 4 ... join ... 6 | contraction | 7; side s0 | projectors, outercore | s1 | type-1 | s2.
+    // it (grid tables, projectors, outercore vectors, type-1), forked after the counters are cleared;
+    // the radial product waits for the grid tables (ev 5) and the contraction joins the rest.
+    // Events: caller 0 start = 1 | screening | 2 | radial | 3 | canonical | 4 ... join ... 6 |
+    // contraction | 7; side s0 | tables | 5 | projectors, outercore | s1 | type-1 | s2.
     HIPCHK(hipMemsetAsync(P->d_counters, 0, 8 * sizeof(unsigned long long), st));
     HIPCHK(hipEventRecord(P->ev[0], st));
     HIPCHK(hipStreamWaitEvent(P->side, P->ev[0], 0));
     hipStream_t sd = P->side;
     HIPCHK(hipEventRecord(P->evs[0], sd));
+    // grid tables first: the radial product waits for them (event ev[5]); the screening does not
+    if (P->n_tab > 0) {
+        dim3 g(NPTS / 256, P->n_tab);
+        hipLaunchKernelGGL(k_tab_F, g, dim3(256), 0, sd, P->d_tab, P->J, P->T, P->d_F);
+    }
+    if (P->n_wpots > 0) {
+        dim3 g(NPTS / 256, P->n_wpots);
+        hipLaunchKernelGGL(k_tab_wU, g, dim3(256), 0, sd, P->d_wpots, P->J, P->T, P->d_wU);
+    }
+    HIPCHK(hipEventRecord(P->ev[5], sd));
     if (P->n_proj > 0) hipLaunchKernelGGL(k_proj, dim3(P->n_proj), dim3(256), 0, sd, P->d_proj, P->J, P->T, P->d_P);
     if (P->n_delta_centers > 0)
         hipLaunchKernelGGL(k_delta, dim3(P->n_delta_centers), dim3(64), 0, sd, P->d_delta_centers,
@@ -2332,18 +2343,11 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
                                P->T, P->d_partial, P->parts, P->t1_scratch);
     }
     HIPCHK(hipEventRecord(P->evs[2], sd));
-    if (P->n_tab > 0) {
-        dim3 g(NPTS / 256, P->n_tab);
-        hipLaunchKernelGGL(k_tab_F, g, dim3(256), 0, st, P->d_tab, P->J, P->T, P->d_F);
-    }
-    if (P->n_wpots > 0) {
-        dim3 g(NPTS / 256, P->n_wpots);
-        hipLaunchKernelGGL(k_tab_wU, g, dim3(256), 0, st, P->d_wpots, P->J, P->T, P->d_wU);
-    }
     HIPCHK(hipEventRecord(P->ev[1], st));
     if (P->n_tiles > 0)
         hipLaunchKernelGGL(k_screen, dim3(4 * P->n_tiles), dim3(64), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->J, P->T,
                            P->d_scr, P->d_counters);
+    HIPCHK(hipStreamWaitEvent(st, P->ev[5], 0));   // grid tables of the side stream
     HIPCHK(hipEventRecord(P->ev[2], st));
     if (P->n_tiles > 0) {
         if (P->radial_mfma && P->d_lev)
@@ -2376,8 +2380,8 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(P->ev[7], st));
     HIPCHK(hipEventSynchronize(P->ev[7]));
-    const hipEvent_t *pb[7] = {&P->ev[0], &P->ev[1], &P->ev[2], &P->ev[3], &P->evs[0], &P->evs[1], &P->ev[6]};
-    const hipEvent_t *pe[7] = {&P->ev[1], &P->ev[2], &P->ev[3], &P->ev[4], &P->evs[1], &P->evs[2], &P->ev[7]};
+    const hipEvent_t *pb[7] = {&P->evs[0], &P->ev[1], &P->ev[2], &P->ev[3], &P->ev[5], &P->evs[1], &P->ev[6]};
+    const hipEvent_t *pe[7] = {&P->ev[5], &P->ev[2], &P->ev[3], &P->ev[4], &P->evs[1], &P->evs[2], &P->ev[7]};
     for (int i = 0; i < 7; i++) {
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, *pb[i], *pe[i]));
