@@ -108,6 +108,21 @@ def test_per_call_outercore_and_full_match_oracle(G, orc, h2o):
         _check(np.array(api.full(A, B, C, g)), orc.per_call("full", A, B, C, g), f"full {a},{b},{c}")
 
 
+def test_per_call_type1_all_powers(G, orc):
+    """type-1 with r^-2, r^-1 and r^0 Gaussian terms (n = 0, 1, 2: the G_n / Dawson, Boys and plain
+    Gaussian branches of type1_mcmurchie_davidson.c), small and large |Q - C| (Dawson beyond
+    q|QC|^2 = 12), l up to 4, against the oracle"""
+    rng = np.random.default_rng(5)
+    api = G.Api()
+    pot = io.Potential(0, 0, [0, 1, 2, 0, 1], [1.7, -2.3, 0.9, -0.4, 3.1], [0.35, 1.2, 4.0, 12.0, 0.08])
+    C = (0.0, 0.0, 0.0)
+    for la, lb, ra, rb in [(0, 0, (0.1, 0, 0), (0, 0.2, 0)), (2, 1, (1.5, -0.5, 0.3), (-2.0, 1.0, 0.5)),
+                           (4, 3, (3.0, 2.0, -1.0), (2.5, -3.0, 1.5)), (3, 3, (6.0, 0, 0), (0, 6.5, 0))]:
+        A = configs._shell(rng, la, ra, 3, (0.2, 15.0))
+        B = configs._shell(rng, lb, rb, 3, (0.2, 15.0))
+        _check(np.asarray(api.type1(A, B, C, pot)), np.asarray(orc.per_call("type1", A, B, C, pot)), f"type1 ({la},{lb})")
+
+
 def test_per_call_high_angular_momentum(G, orc):
     """shells up to l = 8 and channels up to L = 5 (the sizes of the largest workloads) near the GRPP
     centre, per-call type-1 / type-2 / spin-orbit against the oracle (the oracle's own cost bounds
