@@ -252,6 +252,16 @@ def test_full_size_properties(G):
         assert np.abs(a[k] - sgn * a[k].T).max() <= 1e-4 * scale, nm
 
 
+def test_heavy_cluster_gradients_translational_invariance(G):
+    """full heavy_cluster (8 atoms, l <= 4 plus the l + 1 shells of the derivatives): the sum over
+    all atoms of d/dR_atom vanishes to rounding, relative to the largest gradient element"""
+    Gr, st = G.batch_gradients(configs.heavy_cluster())
+    for op in range(4):
+        scale = np.abs(Gr[op]).max()
+        for k in range(3):
+            assert np.abs(Gr[op, k::3].sum(axis=0)).max() <= 1e-13 * scale, (op, k)
+
+
 def test_invalid_input_reports_error(G):
     """shells beyond the supported angular momentum are rejected on the host with a message"""
     bad = io.System([io.Shell(11, (0, 0, 0), [1.0], [1.0])],
