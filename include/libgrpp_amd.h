@@ -82,6 +82,20 @@ int libgrpp_batch_plan_create(int nshells, libgrpp_shell_t *const *shells, int n
                               libgrpp_grpp_t *const *grpp_per_center, int flags, libgrpp_batch_plan_t **plan);
 int libgrpp_batch_plan_execute(libgrpp_batch_plan_t *plan, void *stream);
 int libgrpp_batch_plan_download(libgrpp_batch_plan_t *plan, double *arep, double *so_x, double *so_y, double *so_z);
+
+/*
+ * Many independent molecules in one plan (uneven batches of shell pairs x centres).  Molecule m
+ * owns shells [shell_off[m], shell_off[m+1]) and centres [center_off[m], center_off[m+1]) of the
+ * concatenated lists; its centres act on its shells only.  Its four dim_m x dim_m matrices
+ * (dim_m = sum of cart_size over its shells) follow those of molecule m-1: download_flat writes
+ * out[op * output_len + sum_{k<m} dim_k^2 + i * dim_m + j], op = arep, so_x, so_y, so_z.
+ */
+int libgrpp_batch_molecules_plan_create(int nmol, const int *shell_off, const int *center_off,
+                                        libgrpp_shell_t *const *shells, const double *center_xyz,
+                                        libgrpp_grpp_t *const *grpp_per_center, int flags,
+                                        libgrpp_batch_plan_t **plan);
+long long libgrpp_batch_plan_output_len(const libgrpp_batch_plan_t *plan);   /* doubles per operator */
+int libgrpp_batch_plan_download_flat(libgrpp_batch_plan_t *plan, double *out);   /* 4 * output_len doubles */
 int libgrpp_batch_plan_dim(const libgrpp_batch_plan_t *plan);
 int libgrpp_batch_plan_stats(const libgrpp_batch_plan_t *plan, libgrpp_batch_stats_t *stats);
 /*

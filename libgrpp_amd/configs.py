@@ -11,6 +11,8 @@ Contracted coefficients are normalised with the semantics of libgrpp's
   with 3 primitives, exponents log-uniform [0.05, 20], coefficients uniform [0.1, 1].
   Draws: channels in the order local, AREP L=0..3, SO L=1..3 (zeta then c per channel); then
   shells atom A then atom B, each shell exponents then coefficients.
+* ``ragged_molecules`` -- a list of independent molecules of uneven size (see the function);
+  seed 0 with 200 molecules gives 11,051 shells and 2.70e6 shell-pair x centre tasks.
 * ``heavy_cluster`` -- 8 atoms uniform in a 12-bohr cube, minimum separation 3 bohr (rejection,
   one coordinate triple at a time); per atom, in atom order: GRPP with local (L=5) + semilocal
   L=0..4 and SO L=1..4, 5 terms each as above; two outercore projector shells (s, J=1; p, J=3)
@@ -89,6 +91,44 @@ def heavy_cluster(seed: int = 0, natom: int = 8, box: float = 12.0, lmax_pot: in
             for _ in range(count):
                 shells.append(_shell(rng, L, origin, nprim, alpha))
     return System(shells, centers)
+
+
+def ragged_molecules(seed: int = 0, nmol: int = 200, atoms=(2, 30), centers=(1, 6), lmax_pot=5,
+                     shells_per_atom=(2, 5), lmax_shell: int = 6, nprim=(1, 8), box_per_atom: float = 2.5,
+                     alpha=(0.03, 50.0), oc=((0, 1), (1, 3))) -> List[System]:
+    """Uneven batch of independent molecules: atom count U{atoms}, GRPP centres U{centers} (the
+    first atoms of the molecule) with semilocal L = 0..U{0..lmax_pot} plus local, SO for L >= 1 and
+    outercore s/p projectors (channel and exponent distributions as heavy_cluster), per atom
+    U{shells_per_atom} Cartesian shells with l in 0..lmax_shell drawn with probability proportional
+    to 1/(l+1) and U{nprim} primitives; atoms uniform in a cube of edge box_per_atom * natom^(1/3)
+    * 2 with minimum separation 3 bohr (rejection sampling).  Draw order: per molecule, atom count,
+    positions, centre count, then per centre its GRPP, then per atom its shells."""
+    rng = np.random.default_rng(seed)
+    pl = np.array([1.0 / (l + 1) for l in range(lmax_shell + 1)])
+    pl /= pl.sum()
+    out = []
+    for _ in range(nmol):
+        na = int(rng.integers(atoms[0], atoms[1] + 1))
+        xyz = _positions(rng, na, box_per_atom * 2.0 * na ** (1.0 / 3.0), 3.0)
+        nc = int(rng.integers(centers[0], min(centers[1], na) + 1))
+        cents = []
+        for a in range(nc):
+            g = GRPP()
+            Lmax = int(rng.integers(0, lmax_pot + 1))
+            g.local = _channel(rng, Lmax + 1, 0, 5)
+            g.arep = [_channel(rng, L, 0, 5) for L in range(Lmax + 1)]
+            g.esop = [_channel(rng, L, 0, 5) for L in range(1, Lmax + 1)]
+            for L, J in oc:
+                g.oc.append((_channel(rng, L, J, 5), _shell(rng, L, (0.0, 0.0, 0.0), 4, (0.5, 30.0))))
+            cents.append((tuple(float(x) for x in xyz[a]), g))
+        shells = []
+        for a in range(na):
+            for _ in range(int(rng.integers(shells_per_atom[0], shells_per_atom[1] + 1))):
+                l = int(rng.choice(lmax_shell + 1, p=pl))
+                shells.append(_shell(rng, l, tuple(float(x) for x in xyz[a]), int(rng.integers(nprim[0], nprim[1] + 1)),
+                                     alpha))
+        out.append(System(shells, cents))
+    return out
 
 
 WORKLOADS = {

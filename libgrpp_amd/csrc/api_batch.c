@@ -273,6 +273,7 @@ struct libgrpp_batch_plan {
     eng_plan_t *eng;
     int dim;
     int flags;
+    long long out_len;   /* doubles per operator in the engine's output */
 };
 
 static int check_canonical(int n, libgrpp_shell_t *const *shells)
@@ -352,9 +353,97 @@ int libgrpp_batch_plan_create(int nshells, libgrpp_shell_t *const *shells, int n
     p->eng = eng;
     p->dim = dim;
     p->flags = flags;
+    p->out_len = n2;
     *plan = p;
     return 0;
 }
+
+int libgrpp_batch_molecules_plan_create(int nmol, const int *shell_off, const int *center_off,
+                                        libgrpp_shell_t *const *shells, const double *xyz,
+                                        libgrpp_grpp_t *const *grpps, int flags, libgrpp_batch_plan_t **plan)
+{
+    *plan = NULL;
+    if (nmol <= 0) {
+        eng_set_error("batch_molecules: no molecules");
+        return 2;
+    }
+    const int nshells = shell_off[nmol], ncenters = center_off[nmol];
+    for (int m = 0; m < nmol; m++)
+        if (shell_off[m + 1] < shell_off[m] || center_off[m + 1] < center_off[m]) {
+            eng_set_error("batch_molecules: molecule %d has decreasing shell or centre offsets", m);
+            return 2;
+        }
+    if (!check_canonical(nshells, shells)) return 2;
+    int *cidx = (int *) malloc(sizeof(int) * (ncenters > 0 ? ncenters : 1));
+    int nidx = 0;
+    eng_center_t *cs = make_centers(ncenters, xyz, grpps, cidx, &nidx);
+    /* each centre acts on its own molecule's shells; the centre-index list stays in molecule order */
+    int *cb = (int *) malloc(sizeof(int) * (nmol + 1));
+    int k = 0;
+    for (int m = 0; m < nmol; m++) {
+        cb[m] = k;
+        for (int c = center_off[m]; c < center_off[m + 1]; c++) {
+            cs[c].shell_begin = shell_off[m];
+            cs[c].shell_end = shell_off[m + 1];
+        }
+        while (k < nidx && cidx[k] < center_off[m + 1]) k++;
+    }
+    cb[nmol] = k;
+    int sym = !(flags & LIBGRPP_BATCH_NO_SYMMETRY);
+    long long nit = 0, total = 0;
+    for (int m = 0; m < nmol; m++) {
+        const long long n = shell_off[m + 1] - shell_off[m];
+        nit += sym ? n * (n + 1) / 2 : n * n;
+    }
+    eng_item_t *it = (eng_item_t *) malloc(sizeof(eng_item_t) * (nit > 0 ? nit : 1));
+    int *off = (int *) malloc(sizeof(int) * (nshells + 1));
+    long long q = 0;
+    for (int m = 0; m < nmol; m++) {
+        int dim = 0;
+        for (int i = shell_off[m]; i < shell_off[m + 1]; i++) {
+            off[i] = dim;
+            dim += shells[i]->cart_size;
+        }
+        for (int i = shell_off[m]; i < shell_off[m + 1]; i++)
+            for (int j = sym ? i : shell_off[m]; j < shell_off[m + 1]; j++) {
+                eng_item_t e;
+                e.a = i;
+                e.b = j;
+                e.cb = cb[m];
+                e.ce = cb[m + 1];
+                e.off = total + (long long) off[i] * dim + off[j];
+                e.moff = total + (long long) off[j] * dim + off[i];
+                e.stride = dim;
+                e.mirror = sym && (i != j);
+                it[q++] = e;
+            }
+        total += (long long) dim * dim;
+    }
+    eng_plan_t *eng = NULL;
+    int eflags = ((flags & LIBGRPP_BATCH_LEARNED_GRID) ? ENG_F_LEARNED : 0) |
+                 ((flags & LIBGRPP_BATCH_RECORD_LEVELS) ? ENG_F_LEVELS : 0);
+    int rc = nit > 0x7fffffffLL ? 2
+                                : eng_plan_create(nshells, shells, ncenters, cs, nidx, cidx, (int) nit, it, total, total,
+                                                  ENG_ALL, eflags, &eng);
+    if (nit > 0x7fffffffLL) eng_set_error("batch_molecules: too many shell pairs");
+    free(off);
+    free(cidx);
+    free(cs);
+    free(cb);
+    free(it);
+    if (rc) return rc;
+    libgrpp_batch_plan_t *p = (libgrpp_batch_plan_t *) calloc(1, sizeof(*p));
+    p->eng = eng;
+    p->dim = 0;
+    p->flags = flags;
+    p->out_len = total;
+    *plan = p;
+    return 0;
+}
+
+long long libgrpp_batch_plan_output_len(const libgrpp_batch_plan_t *p) { return p->out_len; }
+
+int libgrpp_batch_plan_download_flat(libgrpp_batch_plan_t *p, double *out) { return eng_plan_download(p->eng, out); }
 
 int libgrpp_batch_plan_execute(libgrpp_batch_plan_t *p, void *stream) { return eng_plan_execute(p->eng, stream); }
 

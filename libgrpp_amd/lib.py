@@ -200,6 +200,54 @@ class Plan:
         self.o.close()
 
 
+class MoleculesPlan:
+    """Many independent molecules in one device-resident plan (libgrpp_batch_molecules_plan_create)."""
+
+    def __init__(self, systems, flags: int = 0):
+        from .io import System
+        self.systems = list(systems)
+        shells = [s for m in self.systems for s in m.shells]
+        centers = [c for m in self.systems for c in m.centers]
+        self.whole = System(shells, centers)
+        self.o = _Objects(self.whole)
+        so = np.cumsum([0] + [len(m.shells) for m in self.systems]).astype(np.int32)
+        co = np.cumsum([0] + [len(m.centers) for m in self.systems]).astype(np.int32)
+        L = lib()
+        L.libgrpp_batch_plan_output_len.restype = C.c_longlong
+        self.h = C.c_void_p()
+        rc = L.libgrpp_batch_molecules_plan_create(len(self.systems), so.ctypes.data_as(C.POINTER(C.c_int)),
+                                                   co.ctypes.data_as(C.POINTER(C.c_int)), self.o.shells,
+                                                   as_dptr(self.o.xyz), self.o.grpps, flags, C.byref(self.h))
+        _check(rc, "libgrpp_batch_molecules_plan_create")
+        self.out_len = L.libgrpp_batch_plan_output_len(self.h)
+
+    def execute(self, stream: int = 0):
+        _check(lib().libgrpp_batch_plan_execute(self.h, C.c_void_p(stream)), "libgrpp_batch_plan_execute")
+
+    def download(self):
+        """per molecule: [arep, so_x, so_y, so_z], each dim_m x dim_m"""
+        flat = np.zeros(4 * self.out_len)
+        _check(lib().libgrpp_batch_plan_download_flat(self.h, as_dptr(flat)), "libgrpp_batch_plan_download_flat")
+        flat = flat.reshape(4, self.out_len)
+        out, o = [], 0
+        for m in self.systems:
+            d = m.dim
+            out.append([flat[k, o:o + d * d].reshape(d, d).copy() for k in range(4)])
+            o += d * d
+        return out
+
+    def stats(self):
+        st = Stats()
+        lib().libgrpp_batch_plan_stats(self.h, C.byref(st))
+        return st.as_dict()
+
+    def close(self):
+        if self.h:
+            lib().libgrpp_batch_plan_destroy(self.h)
+            self.h = C.c_void_p()
+        self.o.close()
+
+
 # ---------------------------------------------------------------- per-call API (libgrpp.h:123-231)
 def _grad_arrays(na: int, nb: int):
     arrs = [np.zeros(na * nb) for _ in range(3)]

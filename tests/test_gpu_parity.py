@@ -262,6 +262,30 @@ def test_heavy_cluster_gradients_translational_invariance(G):
             assert np.abs(Gr[op, k::3].sum(axis=0)).max() <= 1e-13 * scale, (op, k)
 
 
+def test_ragged_molecules_in_one_plan(G, orc):
+    """uneven batch of six small molecules (2-5 atoms, 1-3 GRPP centres with L up to 5, shells up to
+    l = 4) in one plan: every molecule's matrices equal those of its own plan, and the first
+    molecule matches the oracle (parity rule, or 10x the formulation's rounding spread)"""
+    mols = configs.ragged_molecules(seed=3, nmol=6, atoms=(2, 5), centers=(1, 3), lmax_shell=4, nprim=(1, 3),
+                                    shells_per_atom=(1, 3), alpha=(0.3, 20.0))
+    plan = G.MoleculesPlan(mols)
+    plan.execute()
+    both = plan.download()
+    st = plan.stats()
+    plan.close()
+    assert st["n_tasks"] == sum(len(m.shells) ** 2 * len(m.centers) for m in mols)
+    for m, got in zip(mols, both):
+        alone = G.batch_matrices(m)
+        for k in range(4):
+            assert np.abs(got[k] - alone[k]).max() <= 1e-12 * max(1.0, np.abs(alone[k]).max()), k
+    import oracle
+    R, R2 = orc.matrices(mols[0]), oracle.oracle_nofma().matrices(mols[0])
+    for k, nm in enumerate(OPS):
+        spread = np.abs(R[k] - R2[k]).max()
+        print(f"molecule 0 {nm}: max abs {np.abs(both[0][k] - R[k]).max():.3e}, spread {spread:.3e}", flush=True)
+        assert within_noise(both[0][k], R[k], spread).all(), nm
+
+
 def test_invalid_input_reports_error(G):
     """shells beyond the supported angular momentum are rejected on the host with a message"""
     bad = io.System([io.Shell(11, (0, 0, 0), [1.0], [1.0])],
