@@ -29,6 +29,19 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix), AMD published
 
 
+def workload_config(name, sysm, ntask, nint):
+    if isinstance(sysm, list):
+        return {"workload": f"{name}: {len(sysm)} independent molecules of 2-30 atoms in one plan, 1-6 GRPP "
+                            f"centres each (L<=5 + local + SO + OC s/p), Cartesian shells l=0..6 with P ~ 1/(l+1), "
+                            f"1-8 primitives",
+                "molecules": len(sysm), "shells": sum(len(m.shells) for m in sysm),
+                "centres": sum(len(m.centers) for m in sysm), "tasks": ntask, "integrals_per_step": nint, "seed": 0}
+    return {"workload": f"{name}: 8 atoms, 12 bohr cube, GRPP L<=4 + local + SO L<=4 + OC s/p, "
+                        f"basis 4s4p3d2f1g x 6 primitives",
+            "shells": len(sysm.shells), "centres": len(sysm.centers), "dim": sysm.dim, "tasks": ntask,
+            "integrals_per_step": nint, "seed": 0}
+
+
 def traffic_bytes(kernel: str):
     """HBM bytes per launch of ``kernel`` on this workload, from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads for a plain copy of
@@ -52,6 +65,23 @@ def cpu_baseline(sysm, budget_s: float = 15.0):
     if lib is None:
         lib, kind = oracle.oracle(), "port"
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    if isinstance(sysm, list):
+        # independent molecules: 8 random ordered shell pairs (all centres) of molecules taken in a
+        # fixed random order until the budget is spent
+        rng = np.random.default_rng(1)
+        ntask, t, used = 0, 0.0, 0
+        for m in rng.permutation(len(sysm)):
+            mol = sysm[int(m)]
+            ns = len(mol.shells)
+            pairs = np.sort(rng.permutation(ns * ns)[:8])
+            t += lib.matrices(mol, nthreads=cores, pairs=pairs)[4]
+            ntask += len(pairs) * len(mol.centers)
+            used += 1
+            if t >= 0.8 * budget_s:
+                break
+        return {"value": ntask / t, "unit": "tasks/s", "cores": cores, "kind": kind,
+                "sample": f"8 random ordered shell pairs x all centres of {used} molecules = {ntask} tasks in "
+                          f"{t:.1f} s"}
     ns = len(sysm.shells)
     npairs = ns * ns
     rng = np.random.default_rng(1)
@@ -97,7 +127,9 @@ def main():
         return
     from libgrpp_amd import configs, lib as G
 
+    from libgrpp_amd import learned as LG
     sysm = configs.WORKLOADS[args.workload]()
+    molecules = isinstance(sysm, list)
     phases = ["tables_seconds", "screen_seconds", "radial_seconds", "canon_seconds", "proj_seconds",
               "type1_seconds", "contract_seconds", "device_seconds"]
 
@@ -118,7 +150,7 @@ def main():
         return wall, plan.stats(), {k: float(np.mean(v)) for k, v in times.items()}
 
     # plain adaptive method
-    plan = G.Plan(sysm, 0)
+    plan = LG.make_plan(sysm, 0)
     wall_plain, st_plain, mean_plain = timed(plan)
     plain_out = plan.download()
     plan.close()
@@ -127,12 +159,11 @@ def main():
     learned_info = None
     wall, st, mean = wall_plain, st_plain, mean_plain
     if not args.no_learned:
-        from libgrpp_amd import learned as LG
         t0 = time.perf_counter()
-        X, T = LG.training_set(LG.default_training_systems(4))
+        X, T = LG.training_set(LG.default_training_systems(4, args.workload))
         net, loss = LG.train_predictor(X, T, epochs=30)
         t_train = time.perf_counter() - t0
-        plan = G.Plan(sysm, 0)
+        plan = LG.make_plan(sysm, 0)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         LG.apply(plan, net, bf16=True)
@@ -142,8 +173,11 @@ def main():
         out = plan.download()
         plan.close()
         net.close()
-        diff = max(float(np.abs(a - b).max()) for a, b in zip(out, plain_out))
-        learned_info = {"plain_tasks_per_s": configs.n_tasks(sysm) / (wall_plain / args.steps),
+        if molecules:
+            diff = max(float(np.abs(a - b).max()) for ma, mb in zip(out, plain_out) for a, b in zip(ma, mb))
+        else:
+            diff = max(float(np.abs(a - b).max()) for a, b in zip(out, plain_out))
+        learned_info = {"plain_tasks_per_s": st_plain["n_tasks"] / (wall_plain / args.steps),
                         "plain_ms_per_step": 1e3 * wall_plain / args.steps,
                         "speedup_vs_plain": wall_plain / wall,
                         "radial_points_plain": st_plain["n_radial_points"], "radial_points_learned": st["n_radial_points"],
@@ -152,8 +186,9 @@ def main():
                         "max_abs_diff_vs_plain": diff, "training_rows": int(X.shape[0]),
                         "training_loss": loss, "training_s": t_train, "inference_and_reorder_ms": 1e3 * t_pred}
 
-    ntask = configs.n_tasks(sysm)
-    nint = configs.n_integrals(sysm)
+    ntask = st["n_tasks"]
+    nint = st["n_integrals"]
+    assert ntask == (sum(configs.n_tasks(m) for m in sysm) if molecules else configs.n_tasks(sysm))
     ms = 1e3 * wall / args.steps
     tasks_per_s = ntask / (wall / args.steps)
     scr = mean["screen_seconds"]
@@ -174,10 +209,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: 8 atoms, 12 bohr cube, GRPP L<=4 + local + SO L<=4 + OC s/p, "
-                               f"basis 4s4p3d2f1g x 6 primitives",
-                   "shells": len(sysm.shells), "centres": len(sysm.centers), "dim": sysm.dim, "tasks": ntask,
-                   "integrals_per_step": nint, "seed": 0},
+        "config": workload_config(args.workload, sysm, ntask, nint),
         "integrals_per_s": nint / (wall / args.steps),
         "device_ms_per_step": 1e3 * mean["device_seconds"],
         "kernel_ms": {k.replace("_seconds", ""): 1e3 * v for k, v in mean.items() if k != "device_seconds"},

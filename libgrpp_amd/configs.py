@@ -134,6 +134,7 @@ def ragged_molecules(seed: int = 0, nmol: int = 200, atoms=(2, 30), centers=(1, 
 WORKLOADS = {
     "small_pair": small_pair,
     "heavy_cluster": heavy_cluster,
+    "ragged_molecules": ragged_molecules,
 }
 
 

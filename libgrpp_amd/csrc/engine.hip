@@ -5,11 +5,12 @@
 //
 //   k_tab_F     F^{C,A}_lambda(r_i): contracted radial functions of every shell A about every
 //               centre C on the level-ordered Log3 grid (radial_type2_integral.c:465-489);
-//   k_tab_wU    w_i U_p(r_i) for every semilocal / spin-orbit channel p (potential.c:70-81);
+//   k_tab_wU    w_i U_p(r_i) for every semilocal / spin-orbit channel p (potential.c:70-81),
+//               kept as sign * sqrt|.| so that both operands of the product carry half of it;
 //   k_screen    the Shaw-Hill envelope estimate of every radial entry (screening.c:193-378), one
 //               wave per entry; it steers the quadrature exactly as in the reference;
 //   k_radial    the radial integrals of ALL shell pairs of a centre and channel as one FP64
-//               product Q = G diag(wU) G^T over grid points, rows (shell, lambda, n_A) with
+//               product Q = G diag(wU) G^T over grid points (G sqrt|wU| times sign(wU) G^T sqrt|wU|), rows (shell, lambda, n_A) with
 //               G = r^{n_A} F; the K loop runs level by level (31, 63, ..., 8191 points) and
 //               applies libgrpp's per-entry nested-doubling stop rule at each level boundary
 //               (radial_type2_integral.c:242-318); tiles stop early when every entry stopped;
@@ -293,7 +294,11 @@ __global__ __launch_bounds__(256) void k_tab_wU(const int *__restrict__ pots, De
     double v = 0.0;
     for (int k = 0; k < P.nterm; k++)
         v += J.tcoef[P.toff + k] * pow(r, J.tpow[P.toff + k] - 2) * exp(-J.tzeta[P.toff + k] * r2);
-    wU[(size_t) blockIdx.y * NPTS + i] = T.w[i] * v;
+    // stored as sign(u) sqrt|u|: the product stages sqrt|u| on both operands and the sign on one,
+    // so that entry (a, b) rounds the same whichever side of a tile each row sits on, and the
+    // results do not depend on the order of the rows
+    const double u = T.w[i] * v;
+    wU[(size_t) blockIdx.y * NPTS + i] = copysign(sqrt(fabs(u)), u);
 }
 
 constexpr int RT = 64;   // radial tile edge (entries)
@@ -741,7 +746,7 @@ __global__ __launch_bounds__(256) void k_radial(const TileItem *__restrict__ til
             for (int q = 0; q < 8; q++) {
                 const int e = tid + 256 * q, r = e >> 5, kk = e & 31, k = k0 + kk;
                 const int fa = sfa[r], fb = sfb[r];
-                As[kk][r] = (fa >= 0) ? F[(size_t) fa * NPTS + k] * rpow[sna[r] * NPTS + k] : 0.0;
+                As[kk][r] = (fa >= 0) ? F[(size_t) fa * NPTS + k] * rpow[sna[r] * NPTS + k] * fabs(wp[k]) : 0.0;
                 Bs[kk][r] = (fb >= 0) ? F[(size_t) fb * NPTS + k] * rpow[snb[r] * NPTS + k] * wp[k] : 0.0;
             }
             __syncthreads();
@@ -887,7 +892,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
             for (int q = 0; q < 8; q++) {
                 const int e = tid + 256 * q, r = e >> 5, kk = e & 31, k = k0 + kk;
                 const int fa = sfa[r], fb = sfb[r];
-                As[kk][r] = (fa >= 0) ? F[(size_t) fa * NPTS + k] * rpow[sna[r] * NPTS + k] : 0.0;
+                As[kk][r] = (fa >= 0) ? F[(size_t) fa * NPTS + k] * rpow[sna[r] * NPTS + k] * fabs(wp[k]) : 0.0;
                 Bs[kk][r] = (fb >= 0) ? F[(size_t) fb * NPTS + k] * rpow[snb[r] * NPTS + k] * wp[k] : 0.0;
             }
             __syncthreads();

@@ -96,10 +96,17 @@ class MLP:
             self.h = C.c_void_p()
 
 
-def row_labels(sysm: System) -> Tuple[np.ndarray, np.ndarray, dict]:
-    """Features and stopping-level labels of every radial row of ``sysm``, from one execution of
-    the plain adaptive method."""
-    plan = G.Plan(sysm, G.BATCH_RECORD_LEVELS)
+def make_plan(workload, flags: int = 0):
+    """a plan for a System, or for a list of Systems (independent molecules in one plan)"""
+    if isinstance(workload, (list, tuple)):
+        return G.MoleculesPlan(workload, flags)
+    return G.Plan(workload, flags)
+
+
+def row_labels(sysm) -> Tuple[np.ndarray, np.ndarray, dict]:
+    """Features and stopping-level labels of every radial row of ``sysm`` (a System or a list of
+    molecules), from one execution of the plain adaptive method."""
+    plan = make_plan(sysm, G.BATCH_RECORD_LEVELS)
     try:
         plan.execute()
         return plan.features(), plan.row_levels(), plan.stats()
@@ -107,7 +114,7 @@ def row_labels(sysm: System) -> Tuple[np.ndarray, np.ndarray, dict]:
         plan.close()
 
 
-def training_set(systems: Iterable[System]) -> Tuple[np.ndarray, np.ndarray]:
+def training_set(systems: Iterable) -> Tuple[np.ndarray, np.ndarray]:
     xs, ts = [], []
     for s in systems:
         x, t, _ = row_labels(s)
@@ -126,8 +133,11 @@ def train_predictor(X: np.ndarray, T: np.ndarray, epochs: int = 40, batch: int =
     return mlp, loss
 
 
-def default_training_systems(n: int = 4):
-    """heavy_cluster-class systems with seeds 1..n (the measured workload uses seed 0)."""
+def default_training_systems(n: int = 4, workload: str = "heavy_cluster"):
+    """training systems of the workload's class with seeds 1..n (the measured workload uses seed 0);
+    for ragged_molecules each system is a batch of 20 molecules"""
+    if workload == "ragged_molecules":
+        return [configs.ragged_molecules(seed=s, nmol=20) for s in range(1, n + 1)]
     return [configs.heavy_cluster(seed=s) for s in range(1, n + 1)]
 
 

@@ -145,7 +145,32 @@ def batch_gradients(sysm: System, flags: int = 0):
     return out, st.as_dict()
 
 
-class Plan:
+class _LevelHooks:
+    """Radial-grid level predictor hooks of a plan handle ``self.h``."""
+
+    def features(self) -> np.ndarray:
+        n = C.c_int()
+        lib().libgrpp_batch_plan_features(self.h, C.byref(n), None)
+        f = np.zeros((n.value, NFEAT), dtype=np.float32)
+        _check(lib().libgrpp_batch_plan_features(self.h, C.byref(n), f.ctypes.data_as(C.POINTER(C.c_float))),
+               "libgrpp_batch_plan_features")
+        return f
+
+    def row_levels(self) -> np.ndarray:
+        n = C.c_int()
+        lib().libgrpp_batch_plan_row_levels(self.h, C.byref(n), None)
+        v = np.zeros((n.value, 3), dtype=np.float32)
+        _check(lib().libgrpp_batch_plan_row_levels(self.h, C.byref(n), v.ctypes.data_as(C.POINTER(C.c_float))),
+               "libgrpp_batch_plan_row_levels")
+        return v
+
+    def set_predictions(self, pred: np.ndarray):
+        pred = np.ascontiguousarray(pred, dtype=np.float32).reshape(-1, 3)
+        _check(lib().libgrpp_batch_plan_set_predictions(self.h, pred.shape[0],
+                                                        pred.ctypes.data_as(C.POINTER(C.c_float))),
+               "libgrpp_batch_plan_set_predictions")
+
+class Plan(_LevelHooks):
     """Packed, device-resident batched build: execute() runs the kernels only."""
 
     def __init__(self, sysm: System, flags: int = 0):
@@ -170,29 +195,6 @@ class Plan:
         lib().libgrpp_batch_plan_stats(self.h, C.byref(st))
         return st.as_dict()
 
-    # ---- radial-grid level predictor hooks
-    def features(self) -> np.ndarray:
-        n = C.c_int()
-        lib().libgrpp_batch_plan_features(self.h, C.byref(n), None)
-        f = np.zeros((n.value, NFEAT), dtype=np.float32)
-        _check(lib().libgrpp_batch_plan_features(self.h, C.byref(n), f.ctypes.data_as(C.POINTER(C.c_float))),
-               "libgrpp_batch_plan_features")
-        return f
-
-    def row_levels(self) -> np.ndarray:
-        n = C.c_int()
-        lib().libgrpp_batch_plan_row_levels(self.h, C.byref(n), None)
-        v = np.zeros((n.value, 3), dtype=np.float32)
-        _check(lib().libgrpp_batch_plan_row_levels(self.h, C.byref(n), v.ctypes.data_as(C.POINTER(C.c_float))),
-               "libgrpp_batch_plan_row_levels")
-        return v
-
-    def set_predictions(self, pred: np.ndarray):
-        pred = np.ascontiguousarray(pred, dtype=np.float32).reshape(-1, 3)
-        _check(lib().libgrpp_batch_plan_set_predictions(self.h, pred.shape[0],
-                                                        pred.ctypes.data_as(C.POINTER(C.c_float))),
-               "libgrpp_batch_plan_set_predictions")
-
     def close(self):
         if self.h:
             lib().libgrpp_batch_plan_destroy(self.h)
@@ -200,7 +202,7 @@ class Plan:
         self.o.close()
 
 
-class MoleculesPlan:
+class MoleculesPlan(_LevelHooks):
     """Many independent molecules in one device-resident plan (libgrpp_batch_molecules_plan_create)."""
 
     def __init__(self, systems, flags: int = 0):

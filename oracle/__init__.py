@@ -109,9 +109,10 @@ _cache = {}
 
 
 def oracle_nofma() -> CpuLib:
-    """The oracle compiled without floating-point contraction: the same algorithm under a second,
-    equally valid rounding.  Its spread against ``oracle()`` measures how far the reference
-    formulation is determined by its own arithmetic on a given input."""
+    """The oracle compiled without floating-point contraction and with the quadrature points summed
+    in reverse order: the same algorithm under a second, equally valid rounding.  Its spread against
+    ``oracle()`` measures how far the reference formulation is determined by its own arithmetic on a
+    given input."""
     if "nofma" not in _cache:
         if not os.path.exists(ORC_NOFMA_SO):
             raise RuntimeError("oracle/liborc_nofma.so is not built (make -C oracle)")

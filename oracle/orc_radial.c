@@ -139,6 +139,20 @@ static int screen_contracted(int l1, int l2, int n, double CA2, double CB2, cons
     return 0;
 }
 
+/* sum of w U F1 F2 r^n over the grid points [b, e), added to s.  ORC_ALT_ORDER (the second oracle
+ * build) takes the points in reverse order and the factors in reverse association: another valid
+ * rounding of the same quadrature, used only to measure how strongly an input amplifies rounding */
+static inline double point_sum(const double *w, const double *U, const double *F1, const double *F2,
+                               const double *rN, int b, int e, double s)
+{
+#ifdef ORC_ALT_ORDER
+    for (int i = e - 1; i >= b; i--) s += rN[i] * (F2[i] * (F1[i] * (U[i] * w[i])));
+#else
+    for (int i = b; i < e; i++) s += w[i] * U[i] * F1[i] * F2[i] * rN[i];
+#endif
+    return s;
+}
+
 /* adaptive nested-doubling quadrature of one entry (radial_type2_integral.c:242-318) */
 static double integrate_entry(grid_t *g, int n, int l1, int l2, int *level_out)
 {
@@ -158,7 +172,7 @@ static double integrate_entry(grid_t *g, int n, int l1, int l2, int *level_out)
     const double *F1 = g->F1 + l1 * GRID_TOTAL, *F2 = g->F2 + l2 * GRID_TOTAL, *rN = g->rN + n * GRID_TOTAL;
     int nr = MIN_GRID, level = 0;
     double sum = 0.0, prev;
-    for (int i = 0; i < nr; i++) sum += w[i] * U[i] * F1[i] * F2[i] * rN[i];
+    sum = point_sum(w, U, F1, F2, rN, 0, nr, sum);
     if (o->fixed_level == 0) { if (level_out) *level_out = 0; return sum; }
     int converged = 0;
     do {
@@ -168,7 +182,7 @@ static double integrate_entry(grid_t *g, int n, int l1, int l2, int *level_out)
         prev = sum;
         sum = 0.5 * sum;
         grid_expand(g, nr);
-        for (int i = idx; i < nr; i++) sum += w[i] * U[i] * F1[i] * F2[i] * rN[i];
+        sum = point_sum(w, U, F1, F2, rN, idx, nr, sum);
         level++;
         if (o->fixed_level >= 0) {
             if (level >= o->fixed_level) break;

@@ -1,6 +1,6 @@
 #!/usr/bin/env python3
 """Writes KCs/gradient_spread.json: the rounding spread of the reference formulation on the KCs
-gradient matrices kept as goldens, i.e. max |oracle - oracle without FMA contraction| per matrix
+gradient matrices kept as goldens, i.e. max |oracle - second-rounding oracle| per matrix
 (oracle/liborc.so vs oracle/liborc_nofma.so; about 4 minutes on 8 cores).  The GPU tests bound the
 engine's distance to these goldens by a multiple of it.
 

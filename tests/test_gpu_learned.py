@@ -64,7 +64,7 @@ def test_training_fits_a_smooth_target(L):
 
 def test_learned_order_gains_and_keeps_results(L):
     """trained on heavy_cluster seeds 1-2, applied to seed 0: fewer radial grid points than the plain
-    method, the same matrices (the order of rows only changes rounding)"""
+    method, bitwise the same matrices (every radial entry rounds the same in any row order)"""
     from libgrpp_amd import lib as G
     X, T = L.training_set(L.default_training_systems(2))
     net, loss = L.train_predictor(X, T, epochs=30)
@@ -82,4 +82,28 @@ def test_learned_order_gains_and_keeps_results(L):
     print(f"radial grid points: plain {pts_plain}, learned order {pts_learned}; training loss {loss:.3f}")
     assert pts_learned < pts_plain
     for a, b in zip(plain, learned):
-        assert np.abs(a - b).max() <= 1e-6 * max(1.0, np.abs(a).max())
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("workload", ["small_pair", "ragged"])
+def test_any_row_order_gives_identical_matrices(workload):
+    """random predictions (an arbitrary order of the rows of every radial table) leave every matrix
+    bitwise unchanged, also on ill-conditioned molecules where a one-ulp change of a radial entry
+    moves the result far beyond the parity tolerance"""
+    from libgrpp_amd import lib as G
+    if workload == "small_pair":
+        plan = G.Plan(configs.small_pair())
+    else:
+        plan = G.MoleculesPlan(configs.ragged_molecules(seed=0, nmol=200)[158:162])
+    plan.execute()
+    plain = plan.download()
+    n = plan.features().shape[0]
+    rng = np.random.default_rng(7)
+    for _ in range(2):
+        plan.set_predictions(rng.random((n, 3)).astype(np.float32))
+        plan.execute()
+        got = plan.download()
+        flat = lambda ms: [m for x in ms for m in (x if isinstance(x, list) else [x])]
+        for a, b in zip(flat(plain), flat(got)):
+            assert np.array_equal(a, b)
+    plan.close()
