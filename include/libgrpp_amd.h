@@ -25,7 +25,9 @@ enum {
     LIBGRPP_BATCH_LEARNED_GRID = 1,   /* predicted fixed radial grids + embedded-rule check + adaptive fallback */
     LIBGRPP_BATCH_GRADIENT = 2,       /* plan also evaluates nuclear gradients (see ..._gradients) */
     LIBGRPP_BATCH_NO_SYMMETRY = 4,    /* evaluate every ordered pair instead of mirroring B,A from A,B */
-    LIBGRPP_BATCH_RECORD_LEVELS = 8   /* plan records each radial entry's stopping level (training labels) */
+    LIBGRPP_BATCH_RECORD_LEVELS = 8,  /* plan records each radial entry's stopping level (training labels) */
+    LIBGRPP_BATCH_GRAPH = 16          /* execute replays one hipGraph of the whole build, captured by the
+                                         first execute; per-phase times are then not measured (zero) */
 };
 
 typedef struct {

@@ -342,7 +342,8 @@ int libgrpp_batch_plan_create(int nshells, libgrpp_shell_t *const *shells, int n
     eng_plan_t *eng = NULL;
     long long n2 = (long long) dim * dim;
     int eflags = ((flags & LIBGRPP_BATCH_LEARNED_GRID) ? ENG_F_LEARNED : 0) |
-                 ((flags & LIBGRPP_BATCH_RECORD_LEVELS) ? ENG_F_LEVELS : 0);
+                 ((flags & LIBGRPP_BATCH_RECORD_LEVELS) ? ENG_F_LEVELS : 0) |
+                 ((flags & LIBGRPP_BATCH_GRAPH) ? ENG_F_GRAPH : 0);
     int rc = eng_plan_create(nshells, shells, ncenters, cs, nidx, cidx, nit, it, n2, n2, ENG_ALL, eflags, &eng);
     free(off);
     free(cidx);
@@ -421,7 +422,8 @@ int libgrpp_batch_molecules_plan_create(int nmol, const int *shell_off, const in
     }
     eng_plan_t *eng = NULL;
     int eflags = ((flags & LIBGRPP_BATCH_LEARNED_GRID) ? ENG_F_LEARNED : 0) |
-                 ((flags & LIBGRPP_BATCH_RECORD_LEVELS) ? ENG_F_LEVELS : 0);
+                 ((flags & LIBGRPP_BATCH_RECORD_LEVELS) ? ENG_F_LEVELS : 0) |
+                 ((flags & LIBGRPP_BATCH_GRAPH) ? ENG_F_GRAPH : 0);
     int rc = nit > 0x7fffffffLL ? 2
                                 : eng_plan_create(nshells, shells, ncenters, cs, nidx, cidx, (int) nit, it, total, total,
                                                   ENG_ALL, eflags, &eng);

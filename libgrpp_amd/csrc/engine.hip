@@ -1749,6 +1749,10 @@ struct eng_plan {
     unsigned long long *d_counters = nullptr;
     hipEvent_t ev[8] = {};   // execution phases on the caller's stream: see eng_plan_execute
     hipStream_t side = nullptr;   // second stream (default priority) for the radial-independent kernels
+    hipStream_t cap = nullptr;    // capture stream of a graph plan (default priority)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipEvent_t gev[2] = {};       // around a graph replay
     hipEvent_t evs[3] = {};       // phases on the side stream
     // statistics
     long long n_tasks = 0, n_integrals = 0, n_entries = 0;
@@ -2297,6 +2301,10 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&P->ev[i]));
     for (int i = 0; i < 3; i++) HIPCHK(hipEventCreate(&P->evs[i]));
     HIPCHK(hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking));
+    if (flags & ENG_F_GRAPH) {
+        HIPCHK(hipStreamCreateWithFlags(&P->cap, hipStreamNonBlocking));
+        for (int i = 0; i < 2; i++) HIPCHK(hipEventCreate(&P->gev[i]));
+    }
     {
         // radial quadrature on the matrix cores unless GRPP_RADIAL_KERNEL=valu
         const char *k = getenv("GRPP_RADIAL_KERNEL");
@@ -2306,16 +2314,24 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     return 0;
 }
 
-extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
+static int eager_phase_times(eng_plan_t *P);
+
+__global__ void k_zero_u64(unsigned long long *p, int n)
 {
-    hipStream_t st = (hipStream_t) stream_;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
+
+// Enqueues one whole build on st (and the plan's side stream, forked from and joined back to st).
+// It makes no allocation, memset or synchronisation, so it can be captured into a graph.
+static int enqueue_build(eng_plan_t *P, hipStream_t st)
+{
     // Two streams of default priority.  The caller's stream runs the radial chain (tables,
     // screening, radial product, canonical splits); the side stream runs what does not depend on
     // it (grid tables, projectors, outercore vectors, type-1), forked after the counters are cleared;
     // the radial product waits for the grid tables (ev 5) and the contraction joins the rest.
     // Events: caller 0 start = 1 | screening | 2 | radial | 3 | canonical | 4 ... join ... 6 |
     // contraction | 7; side s0 | tables | 5 | projectors, outercore | s1 | type-1 | s2.
-    HIPCHK(hipMemsetAsync(P->d_counters, 0, 8 * sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, P->d_counters, 8);
     HIPCHK(hipEventRecord(P->ev[0], st));
     HIPCHK(hipStreamWaitEvent(P->side, P->ev[0], 0));
     hipStream_t sd = P->side;
@@ -2336,17 +2352,9 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
                            P->n_delta_centers, P->J, P->d_delta);
     if (P->n_ocv > 0) hipLaunchKernelGGL(k_ocv, dim3(P->n_ocv), dim3(256), 0, sd, P->d_ocvi, P->J, P->T, P->d_OCV);
     HIPCHK(hipEventRecord(P->evs[1], sd));
-    if (P->n_items > 0) {
-        // dynamic LDS beyond the 64 KiB default (up to the 160 KiB of a gfx950 CU) must be opted into
-        HIPCHK(hipFuncSetAttribute((const void *) k_type1, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (P->t1_scratch + P->t1_acc) * (int) sizeof(double)));
-        HIPCHK(hipFuncSetAttribute((const void *) k_contract, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (P->ct_qs + P->ct_ts) * (int) sizeof(double)));
-        if (P->n_parts > 0)
-            hipLaunchKernelGGL(k_type1, dim3(P->n_parts), dim3(256),
-                               (size_t) (P->t1_scratch + P->t1_acc) * sizeof(double), sd, P->d_parts, P->d_items, P->J,
-                               P->T, P->d_partial, P->parts, P->t1_scratch);
-    }
+    if (P->n_parts > 0)
+        hipLaunchKernelGGL(k_type1, dim3(P->n_parts), dim3(256), (size_t) (P->t1_scratch + P->t1_acc) * sizeof(double),
+                           sd, P->d_parts, P->d_items, P->J, P->T, P->d_partial, P->parts, P->t1_scratch);
     HIPCHK(hipEventRecord(P->evs[2], sd));
     HIPCHK(hipEventRecord(P->ev[1], st));
     if (P->n_tiles > 0)
@@ -2383,8 +2391,66 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
         hipLaunchKernelGGL(k_contract_reduce, dim3(P->n_items), dim3(256), 0, st, P->d_items, P->d_item_part0,
                            P->d_parts, P->J, P->d_partial, P->d_out);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(P->ev[7], st));
-    HIPCHK(hipEventSynchronize(P->ev[7]));
+    return 0;
+}
+
+extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
+{
+    hipStream_t st = (hipStream_t) stream_;
+    // dynamic LDS beyond the 64 KiB default (up to the 160 KiB of a gfx950 CU) must be opted into;
+    // the attribute belongs to the kernel, not the plan, so each execution sets its plan's sizes
+    if (P->n_parts > 0) {
+        HIPCHK(hipFuncSetAttribute((const void *) k_type1, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (P->t1_scratch + P->t1_acc) * (int) sizeof(double)));
+        HIPCHK(hipFuncSetAttribute((const void *) k_contract, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (P->ct_qs + P->ct_ts) * (int) sizeof(double)));
+    }
+    if (P->flags & ENG_F_GRAPH) {
+        // the build is captured once, on a plan-owned stream, and replayed on the caller's stream;
+        // kernel arguments are device pointers and sizes fixed at plan creation, so a replay sees
+        // the current contents of every buffer (e.g. a new row order)
+        if (!P->gexec) {
+            HIPCHK(hipStreamBeginCapture(P->cap, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue_build(P, P->cap);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(P->cap, &g);
+            if (rc) {
+                if (g) (void) hipGraphDestroy(g);
+                return rc;
+            }
+            HIPCHK(e);
+            P->graph = g;
+            HIPCHK(hipGraphInstantiate(&P->gexec, P->graph, nullptr, nullptr, 0));
+        }
+        HIPCHK(hipEventRecord(P->gev[0], st));
+        HIPCHK(hipGraphLaunch(P->gexec, st));
+        HIPCHK(hipEventRecord(P->gev[1], st));
+        HIPCHK(hipEventSynchronize(P->gev[1]));
+        for (int i = 0; i < 7; i++) P->phase_seconds[i] = 0.0;   // phases are not timed inside a graph
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, P->gev[0], P->gev[1]));
+        P->last_seconds = ms * 1e-3;
+        P->last_radial_seconds = 0.0;
+    } else {
+        const int rc = enqueue_build(P, st);
+        if (rc) return rc;
+        HIPCHK(hipEventRecord(P->ev[7], st));
+        HIPCHK(hipEventSynchronize(P->ev[7]));
+        const int rt = eager_phase_times(P);
+        if (rt) return rt;
+    }
+    unsigned long long cnt[8];
+    HIPCHK(hipMemcpy(cnt, P->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    P->last_points = cnt[0];
+    P->last_tiles = cnt[1];
+    P->last_entry_points = cnt[2];
+    P->last_screen_triples = cnt[3];
+    P->last_screen_steps = cnt[4];
+    return 0;
+}
+
+static int eager_phase_times(eng_plan_t *P)
+{
     const hipEvent_t *pb[7] = {&P->evs[0], &P->ev[1], &P->ev[2], &P->ev[3], &P->ev[5], &P->evs[1], &P->ev[6]};
     const hipEvent_t *pe[7] = {&P->ev[5], &P->ev[2], &P->ev[3], &P->ev[4], &P->evs[1], &P->evs[2], &P->ev[7]};
     for (int i = 0; i < 7; i++) {
@@ -2396,13 +2462,6 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
     HIPCHK(hipEventElapsedTime(&ms, P->ev[0], P->ev[7]));
     P->last_seconds = ms * 1e-3;
     P->last_radial_seconds = P->phase_seconds[2];
-    unsigned long long cnt[8];
-    HIPCHK(hipMemcpy(cnt, P->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
-    P->last_points = cnt[0];
-    P->last_tiles = cnt[1];
-    P->last_entry_points = cnt[2];
-    P->last_screen_triples = cnt[3];
-    P->last_screen_steps = cnt[4];
     return 0;
 }
 
@@ -2423,6 +2482,11 @@ extern "C" void eng_plan_destroy(eng_plan_t *P)
     for (int i = 0; i < 3; i++)
         if (P->evs[i]) (void) hipEventDestroy(P->evs[i]);
     if (P->side) (void) hipStreamDestroy(P->side);
+    if (P->gexec) (void) hipGraphExecDestroy(P->gexec);
+    if (P->graph) (void) hipGraphDestroy(P->graph);
+    if (P->cap) (void) hipStreamDestroy(P->cap);
+    for (int i = 0; i < 2; i++)
+        if (P->gev[i]) (void) hipEventDestroy(P->gev[i]);
     delete P;
 }
 

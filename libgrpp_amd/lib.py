@@ -34,6 +34,7 @@ BATCH_LEARNED_GRID = 1
 BATCH_GRADIENT = 2
 BATCH_NO_SYMMETRY = 4
 BATCH_RECORD_LEVELS = 8
+BATCH_GRAPH = 16
 NFEAT = 12
 
 
