@@ -121,6 +121,17 @@ void libgrpp_outercore_potential_integrals(libgrpp_shell_t *shell_A, libgrpp_she
                                            libgrpp_potential_t **oc_potentials, libgrpp_shell_t **oc_shells,
                                            double *arep, double *esop_x, double *esop_y,
                                            double *esop_z);                             /* outercore_integrals.c:83 */
+/* the two parts of the outercore operator, each ADDED to the outputs (extern in the reference library,
+ * declared by its Fortran layer, fortran.c:263-267 and :367-372) */
+void libgrpp_outercore_potential_integrals_part_1(libgrpp_shell_t *shell_A, libgrpp_shell_t *shell_B, double *C,
+                                                  libgrpp_potential_t *oc_potential, libgrpp_shell_t *oc_shell,
+                                                  double *arep_matrix, double *so_x_matrix, double *so_y_matrix,
+                                                  double *so_z_matrix);              /* outercore_integrals.c:147 */
+void libgrpp_outercore_potential_integrals_part_2(libgrpp_shell_t *shell_A, libgrpp_shell_t *shell_B, double *C,
+                                                  libgrpp_potential_t *oc_potential_1, libgrpp_shell_t *oc_shell_1,
+                                                  libgrpp_potential_t *oc_potential_2, libgrpp_shell_t *oc_shell_2,
+                                                  double *arep_matrix, double *so_x_matrix, double *so_y_matrix,
+                                                  double *so_z_matrix);              /* outercore_integrals.c:277 */
 void libgrpp_full_grpp_integrals(libgrpp_shell_t *shell_A, libgrpp_shell_t *shell_B,
                                  libgrpp_grpp_t *grpp_operator, double *grpp_origin, double *arep_matrix,
                                  double *so_x_matrix, double *so_y_matrix,

@@ -101,6 +101,17 @@ int libgrpp_batch_plan_download_flat(libgrpp_batch_plan_t *plan, double *out);  
 int libgrpp_batch_plan_dim(const libgrpp_batch_plan_t *plan);
 int libgrpp_batch_plan_stats(const libgrpp_batch_plan_t *plan, libgrpp_batch_stats_t *stats);
 /*
+ * Gradient plan: the inputs of libgrpp_batch_full_grpp_gradients packed once.  execute evaluates
+ * the engine's L-1 / L+1 blocks; download_gradients scatters them into gradient_len doubles laid
+ * out as in libgrpp_batch_full_grpp_gradients.  The level-predictor hooks below apply as well;
+ * libgrpp_batch_plan_download is refused for a gradient plan.
+ */
+int libgrpp_batch_gradients_plan_create(int nshells, libgrpp_shell_t *const *shells, int ncenters,
+                                        const double *center_xyz, libgrpp_grpp_t *const *grpp_per_center,
+                                        int flags, libgrpp_batch_plan_t **plan);
+long long libgrpp_batch_plan_gradient_len(const libgrpp_batch_plan_t *plan);   /* 12 * ncenters * dim^2 */
+int libgrpp_batch_plan_download_gradients(libgrpp_batch_plan_t *plan, double *out);
+/*
  * Radial-grid level predictor hooks.  The plan's radial tables have one row per (centre, channel
  * L, shell, lambda, n_A).  features: *n rows x 12 floats (L, l, lambda, n, |CA|, ln alpha_min,
  * ln alpha_max, ln zeta_min, ln zeta_max, ln(1 + 2 alpha_max |CA|), primitives, on-centre flag);

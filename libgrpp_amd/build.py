@@ -48,6 +48,9 @@ def build(verbose: bool = False) -> str:
         (["gcc", "-O3", "-fPIC", "-std=gnu11", "-Wall"] + inc + ["-c", os.path.join(CSRC, "api_batch.c"), "-o",
                                                                  os.path.join(BUILD, "api_batch.o")],
          os.path.join(BUILD, "api_batch.o")),
+        (["gcc", "-O3", "-fPIC", "-std=gnu11", "-Wall"] + inc + ["-c", os.path.join(CSRC, "fortran.c"), "-o",
+                                                                 os.path.join(BUILD, "fortran.o")],
+         os.path.join(BUILD, "fortran.o")),
     ]
     for cmd, obj in steps:
         out = _run(cmd)

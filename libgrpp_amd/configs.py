@@ -93,6 +93,18 @@ def heavy_cluster(seed: int = 0, natom: int = 8, box: float = 12.0, lmax_pot: in
     return System(shells, centers)
 
 
+def high_l_cluster(seed: int = 0, natom: int = 32, box: float = 20.0, lmax_pot: int = 5, lmax_shell: int = 8,
+                   nprim: int = 8, alpha=(0.02, 80.0), oc=((0, 1), (1, 3), (2, 5))) -> System:
+    """High angular momentum class: natom atoms uniform in a box-bohr cube (minimum separation 3
+    bohr), each with a GRPP of semilocal L = 0..lmax_pot plus local, SO L = 1..lmax_pot and
+    outercore s/p/d projectors (J = 2j = 2l+1), and one Cartesian shell of every l = 0..lmax_shell
+    with nprim primitives, exponents log-uniform in alpha.  Draw order as heavy_cluster: positions,
+    then per atom its GRPP (local, AREP, SO, then per outercore entry shell then channel) and its
+    shells in increasing l."""
+    return heavy_cluster(seed=seed, natom=natom, box=box, lmax_pot=lmax_pot, basis=(1,) * (lmax_shell + 1),
+                         nprim=nprim, alpha=alpha, oc=oc)
+
+
 def ragged_molecules(seed: int = 0, nmol: int = 200, atoms=(2, 30), centers=(1, 6), lmax_pot=5,
                      shells_per_atom=(2, 5), lmax_shell: int = 6, nprim=(1, 8), box_per_atom: float = 2.5,
                      alpha=(0.03, 50.0), oc=((0, 1), (1, 3))) -> List[System]:
@@ -135,6 +147,7 @@ WORKLOADS = {
     "small_pair": small_pair,
     "heavy_cluster": heavy_cluster,
     "ragged_molecules": ragged_molecules,
+    "high_l_cluster": high_l_cluster,
 }
 
 

@@ -262,14 +262,16 @@ static int canon_index(int n, int l, int m)
 }
 
 /* copies engine block (canonical Cartesian order, row-major ncA x ncB) into the caller's order */
+/* the caller's components (cart_list, possibly fewer than the shell's (L+1)(L+2)/2) of a block the
+ * engine computed over all components of both shells in canonical order */
 static void to_shell_order(const libgrpp_shell_t *A, const libgrpp_shell_t *B, const double *src, double *dst)
 {
-    int nA = A->cart_size, nB = B->cart_size;
+    int nA = A->cart_size, nB = B->cart_size, nBf = (B->L + 1) * (B->L + 2) / 2;
     for (int i = 0; i < nA; i++) {
         int ci = canon_index(A->cart_list[3 * i], A->cart_list[3 * i + 1], A->cart_list[3 * i + 2]);
         for (int j = 0; j < nB; j++) {
             int cj = canon_index(B->cart_list[3 * j], B->cart_list[3 * j + 1], B->cart_list[3 * j + 2]);
-            dst[i * nB + j] = src[ci * nB + cj];
+            dst[i * nB + j] = src[ci * nBf + cj];
         }
     }
 }
@@ -296,15 +298,17 @@ static void run_task(const char *who, libgrpp_shell_t *A, libgrpp_shell_t *B, co
                      double *out4)
 {
     libgrpp_shell_t *sh[2] = {A, B};
-    int nA = A->cart_size, nB = B->cart_size, sz = nA * nB;
-    eng_item_t it = {0, 1, 0, 1, 0, 0, nB, 0};
+    /* the engine evaluates every Cartesian component of both shells, whatever their cart lists */
+    int nAf = (A->L + 1) * (A->L + 2) / 2, nBf = (B->L + 1) * (B->L + 2) / 2, szf = nAf * nBf;
+    int sz = A->cart_size * B->cart_size;
+    eng_item_t it = {0, 1, 0, 1, 0, 0, nBf, 0};
     int cidx = 0;
-    double *buf = (double *) malloc(sizeof(double) * 4 * sz);
-    if (eng_run(2, sh, 1, c, 1, &cidx, 1, &it, sz, sz, parts, buf)) {
+    double *buf = (double *) malloc(sizeof(double) * 4 * szf);
+    if (eng_run(2, sh, 1, c, 1, &cidx, 1, &it, szf, szf, parts, buf)) {
         free(buf);
         fail(who);
     }
-    for (int op = 0; op < 4; op++) to_shell_order(A, B, buf + op * sz, out4 + op * sz);
+    for (int op = 0; op < 4; op++) to_shell_order(A, B, buf + op * szf, out4 + op * sz);
     free(buf);
 }
 
@@ -385,6 +389,52 @@ void libgrpp_outercore_potential_integrals(libgrpp_shell_t *A, libgrpp_shell_t *
     memcpy(esop_y, buf + 2 * sz, sizeof(double) * sz);
     memcpy(esop_z, buf + 3 * sz, sizeof(double) * sz);
     free(buf);
+}
+
+/* one outercore part of one task, ADDED to the four outputs as the reference's parts accumulate */
+static void oc_part(const char *who, libgrpp_shell_t *A, libgrpp_shell_t *B, double *C, int n_oc,
+                    libgrpp_potential_t **pots, libgrpp_shell_t **shells, int parts, double *arep, double *so_x,
+                    double *so_y, double *so_z)
+{
+    require_init(who);
+    int sz = A->cart_size * B->cart_size;
+    eng_center_t c;
+    memset(&c, 0, sizeof(c));
+    c.origin = C;
+    c.n_oc = n_oc;
+    c.oc_pot = pots;
+    c.oc_shell = shells;
+    c.shell_end = -1;
+    c.oc_i = 0;   /* part 2: the (first, second) term only */
+    c.oc_j = 1;
+    double *buf = (double *) malloc(sizeof(double) * 4 * sz);
+    run_task(who, A, B, &c, parts, buf);
+    double *outs[4] = {arep, so_x, so_y, so_z};
+    for (int op = 0; op < 4; op++)
+        for (int e = 0; e < sz; e++) outs[op][e] += buf[op * sz + e];
+    free(buf);
+}
+
+/* U|nlj><nlj| + |nlj><nlj|U of one outercore entry (outercore_integrals.c:147-270) */
+void libgrpp_outercore_potential_integrals_part_1(libgrpp_shell_t *A, libgrpp_shell_t *B, double *C,
+                                                  libgrpp_potential_t *oc_potential, libgrpp_shell_t *oc_shell,
+                                                  double *arep, double *so_x, double *so_y, double *so_z)
+{
+    oc_part("libgrpp_outercore_potential_integrals_part_1", A, B, C, 1, &oc_potential, &oc_shell, ENG_OC_P1, arep,
+            so_x, so_y, so_z);
+}
+
+/* -|nlj><nlj| (U + U')/2 |n'lj><n'lj| of one ordered pair of outercore entries, zero unless their
+ * (L, J) agree (outercore_integrals.c:277-390) */
+void libgrpp_outercore_potential_integrals_part_2(libgrpp_shell_t *A, libgrpp_shell_t *B, double *C,
+                                                  libgrpp_potential_t *oc_potential_1, libgrpp_shell_t *oc_shell_1,
+                                                  libgrpp_potential_t *oc_potential_2, libgrpp_shell_t *oc_shell_2,
+                                                  double *arep, double *so_x, double *so_y, double *so_z)
+{
+    libgrpp_potential_t *pots[2] = {oc_potential_1, oc_potential_2};
+    libgrpp_shell_t *shells[2] = {oc_shell_1, oc_shell_2};
+    oc_part("libgrpp_outercore_potential_integrals_part_2", A, B, C, 2, pots, shells, ENG_OC_P2, arep, so_x, so_y,
+            so_z);
 }
 
 void libgrpp_full_grpp_integrals(libgrpp_shell_t *A, libgrpp_shell_t *B, libgrpp_grpp_t *g, double *C,

@@ -102,11 +102,14 @@ static void gradient_task(const char *who, libgrpp_shell_t *A, libgrpp_shell_t *
         eng_set_error("libgrpp_init() has not been called");
         fail(who);
     }
-    int nA = A->cart_size, nB = B->cart_size, sz = nA * nB;
+    /* the engine evaluates every Cartesian component of both shells (canonical order, nA x nB);
+     * the caller's cart lists, possibly shorter, select the outputs at the end */
+    int nA = (A->L + 1) * (A->L + 2) / 2, nB = (B->L + 1) * (B->L + 2) / 2, sz = nA * nB;
+    int nAc = A->cart_size, nBc = B->cart_size;
     double **outs[4] = {ga, gx, gy, gz};
     for (int op = 0; op < 4; op++)
         for (int k = 0; k < 3; k++)
-            if (outs[op]) memset(outs[op][k], 0, sizeof(double) * sz);
+            if (outs[op]) memset(outs[op][k], 0, sizeof(double) * nAc * nBc);
     int AC = points_equal(A->origin, C), BC = points_equal(B->origin, C);
     if (AC && BC) return;
     if (!points_equal(A->origin, D) && !points_equal(B->origin, D) && !points_equal(C, D)) return;
@@ -157,10 +160,11 @@ static void gradient_task(const char *who, libgrpp_shell_t *A, libgrpp_shell_t *
         free(kt);
     }
     int AD = points_equal(A->origin, D), BD = points_equal(B->origin, D), CD = points_equal(C, D);
+    double *gfull = (double *) calloc(12 * sz, sizeof(double));
     for (int op = 0; op < 4; op++) {
         if (!outs[op]) continue;
         for (int k = 0; k < 3; k++) {
-            double *g = outs[op][k];
+            double *g = gfull + (op * 3 + k) * sz;
             const double *br = bra + (op * 3 + k) * sz, *kt = ket + (op * 3 + k) * sz;
             if (!AC) {
                 if (AD) for (int e = 0; e < sz; e++) g[e] += br[e];
@@ -172,23 +176,22 @@ static void gradient_task(const char *who, libgrpp_shell_t *A, libgrpp_shell_t *
             }
         }
     }
-    /* caller's Cartesian order */
+    /* caller's Cartesian components and order */
     for (int op = 0; op < 4; op++) {
         if (!outs[op]) continue;
         for (int k = 0; k < 3; k++) {
             double *g = outs[op][k];
-            double *tmp = (double *) malloc(sizeof(double) * sz);
-            memcpy(tmp, g, sizeof(double) * sz);
-            for (int i = 0; i < nA; i++) {
+            const double *full = gfull + (op * 3 + k) * sz;
+            for (int i = 0; i < nAc; i++) {
                 int ci = cidx_of(A->cart_list[3 * i], A->cart_list[3 * i + 1], A->cart_list[3 * i + 2]);
-                for (int j = 0; j < nB; j++) {
+                for (int j = 0; j < nBc; j++) {
                     int cj = cidx_of(B->cart_list[3 * j], B->cart_list[3 * j + 1], B->cart_list[3 * j + 2]);
-                    g[i * nB + j] = tmp[ci * nB + cj];
+                    g[i * nBc + j] = full[ci * nB + cj];
                 }
             }
-            free(tmp);
         }
     }
+    free(gfull);
     free(bra);
     free(ket);
     free(buf);
@@ -269,11 +272,24 @@ void libgrpp_outercore_potential_integrals_gradient(libgrpp_shell_t *A, libgrpp_
 }
 
 /* ---------------------------------------------------------------- batched entry points */
+/* host side of a gradient plan: what the scatter of the engine's L-1 / L+1 blocks needs */
+typedef struct {
+    int nshells, ncenters, nidx;
+    int *L, *nc, *off;        /* per basis shell: angular momentum, Cartesian size, offset */
+    int *ncu, *ncd;           /* Cartesian sizes of the L+1 and L-1 shells (0: none) */
+    int *atom;                /* centre at the shell's origin, or -1 */
+    int *cidx;                /* centres with a GRPP */
+    double *xyz, *origin;     /* centre and shell coordinates */
+    long long *bup, *bdn;     /* block offsets of <i+|U_C|j>, <i-|U_C|j> (-1: none) per (i, j, centre) */
+    long long tot;            /* doubles per operator of the engine's output */
+} grad_host_t;
+
 struct libgrpp_batch_plan {
     eng_plan_t *eng;
     int dim;
     int flags;
     long long out_len;   /* doubles per operator in the engine's output */
+    grad_host_t *grad;   /* gradient plans only */
 };
 
 static int check_canonical(int n, libgrpp_shell_t *const *shells)
@@ -449,8 +465,14 @@ int libgrpp_batch_plan_download_flat(libgrpp_batch_plan_t *p, double *out) { ret
 
 int libgrpp_batch_plan_execute(libgrpp_batch_plan_t *p, void *stream) { return eng_plan_execute(p->eng, stream); }
 
+static void grad_host_free(grad_host_t *g);
+
 int libgrpp_batch_plan_download(libgrpp_batch_plan_t *p, double *arep, double *so_x, double *so_y, double *so_z)
 {
+    if (p->grad) {
+        eng_set_error("download: a gradient plan's output is read by libgrpp_batch_plan_download_gradients");
+        return 2;
+    }
     long long n2 = (long long) p->dim * p->dim;
     double *buf = (double *) malloc(sizeof(double) * 4 * (n2 > 0 ? n2 : 1));
     int rc = eng_plan_download(p->eng, buf);
@@ -477,6 +499,7 @@ void libgrpp_batch_plan_destroy(libgrpp_batch_plan_t *p)
 {
     if (!p) return;
     eng_plan_destroy(p->eng);
+    grad_host_free(p->grad);
     free(p);
 }
 
@@ -497,150 +520,227 @@ int libgrpp_batch_full_grpp_integrals(int nshells, libgrpp_shell_t *const *shell
 /*
  * Batched gradients: the engine evaluates, per centre, the blocks <i-|U_C|j> and <i+|U_C|j> for
  * every ordered pair; ket derivatives reuse them through <i|U|j+-> = s <j+-|U|i>^T
- * (s = +1 arep, -1 spin-orbit).
+ * (s = +1 arep, -1 spin-orbit).  A gradient plan holds the engine plan of those blocks and the
+ * host data of the scatter, so the inputs stay resident between executions.
  */
-int libgrpp_batch_full_grpp_gradients(int nshells, libgrpp_shell_t *const *shells, int ncenters, const double *xyz,
-                                      libgrpp_grpp_t *const *grpps, int flags, double *out,
-                                      libgrpp_batch_stats_t *stats)
+static void grad_host_free(grad_host_t *g)
 {
-    (void) flags;
+    if (!g) return;
+    free(g->L);
+    free(g->nc);
+    free(g->off);
+    free(g->ncu);
+    free(g->ncd);
+    free(g->atom);
+    free(g->cidx);
+    free(g->xyz);
+    free(g->origin);
+    free(g->bup);
+    free(g->bdn);
+    free(g);
+}
+
+int libgrpp_batch_gradients_plan_create(int nshells, libgrpp_shell_t *const *shells, int ncenters, const double *xyz,
+                                        libgrpp_grpp_t *const *grpps, int flags, libgrpp_batch_plan_t **plan)
+{
+    *plan = NULL;
     if (!check_canonical(nshells, shells)) return 2;
-    int *off = (int *) malloc(sizeof(int) * (nshells + 1));
-    off[0] = 0;
-    for (int i = 0; i < nshells; i++) off[i + 1] = off[i] + shells[i]->cart_size;
-    int dim = off[nshells];
-    long long n2 = (long long) dim * dim;
-    memset(out, 0, sizeof(double) * 12 * ncenters * n2);
+    grad_host_t *g = (grad_host_t *) calloc(1, sizeof(*g));
+    const int ns1 = nshells + 1;
+    g->nshells = nshells;
+    g->ncenters = ncenters;
+    g->L = (int *) malloc(sizeof(int) * ns1);
+    g->nc = (int *) malloc(sizeof(int) * ns1);
+    g->off = (int *) malloc(sizeof(int) * ns1);
+    g->ncu = (int *) malloc(sizeof(int) * ns1);
+    g->ncd = (int *) malloc(sizeof(int) * ns1);
+    g->atom = (int *) malloc(sizeof(int) * ns1);
+    g->origin = (double *) malloc(sizeof(double) * 3 * ns1);
+    g->xyz = (double *) malloc(sizeof(double) * 3 * (ncenters > 0 ? ncenters : 1));
+    if (ncenters > 0) memcpy(g->xyz, xyz, sizeof(double) * 3 * ncenters);
+    g->off[0] = 0;
+    for (int i = 0; i < nshells; i++) {
+        g->L[i] = shells[i]->L;
+        g->nc[i] = shells[i]->cart_size;
+        g->off[i + 1] = g->off[i] + g->nc[i];
+        memcpy(g->origin + 3 * i, shells[i]->origin, sizeof(double) * 3);
+        g->atom[i] = -1;
+        for (int c = 0; c < ncenters; c++)
+            if (points_equal(shells[i]->origin, xyz + 3 * c)) {
+                g->atom[i] = c;
+                break;
+            }
+    }
+    const int dim = g->off[nshells];
 
     /* extended shell list: basis, then (down, up) per basis shell */
     int next = nshells;
-    libgrpp_shell_t **sh = (libgrpp_shell_t **) malloc(sizeof(void *) * 3 * (nshells + 1));
-    int *idn = (int *) malloc(sizeof(int) * (nshells + 1)), *iup = (int *) malloc(sizeof(int) * (nshells + 1));
+    libgrpp_shell_t **sh = (libgrpp_shell_t **) malloc(sizeof(void *) * 3 * ns1);
+    int *idn = (int *) malloc(sizeof(int) * ns1), *iup = (int *) malloc(sizeof(int) * ns1);
     for (int i = 0; i < nshells; i++) sh[i] = shells[i];
     for (int i = 0; i < nshells; i++) {
         libgrpp_shell_t *d, *u;
         diff_shell(shells[i], &d, &u);
         idn[i] = d ? next : -1;
+        g->ncd[i] = d ? d->cart_size : 0;
         if (d) sh[next++] = d;
         iup[i] = next;
+        g->ncu[i] = u->cart_size;
         sh[next++] = u;
     }
-    int *cidx = (int *) malloc(sizeof(int) * (ncenters > 0 ? ncenters : 1));
-    int nidx = 0;
-    eng_center_t *cs = make_centers(ncenters, xyz, grpps, cidx, &nidx);
+    g->cidx = (int *) malloc(sizeof(int) * (ncenters > 0 ? ncenters : 1));
+    eng_center_t *cs = make_centers(ncenters, xyz, grpps, g->cidx, &g->nidx);
+    const int nidx = g->nidx;
     /* items: per (i, j, centre) the down and up blocks, op-major in one buffer */
-    long long nitem_max = 2LL * nshells * nshells * (nidx > 0 ? nidx : 1);
-    eng_item_t *it = (eng_item_t *) malloc(sizeof(eng_item_t) * nitem_max);
-    long long *bup = (long long *) malloc(sizeof(long long) * nshells * nshells * (nidx > 0 ? nidx : 1));
-    long long *bdn = (long long *) malloc(sizeof(long long) * nshells * nshells * (nidx > 0 ? nidx : 1));
+    const long long nkey = (long long) nshells * nshells * (nidx > 0 ? nidx : 1);
+    eng_item_t *it = (eng_item_t *) malloc(sizeof(eng_item_t) * 2 * nkey);
+    g->bup = (long long *) malloc(sizeof(long long) * nkey);
+    g->bdn = (long long *) malloc(sizeof(long long) * nkey);
     long long tot = 0;
     int nit = 0;
     for (int i = 0; i < nshells; i++)
         for (int j = 0; j < nshells; j++)
             for (int ci = 0; ci < nidx; ci++) {
-                long long key = ((long long) i * nshells + j) * nidx + ci;
-                int ncb = shells[j]->cart_size;
-                bdn[key] = -1;
+                const long long key = ((long long) i * nshells + j) * nidx + ci;
+                const int ncb = g->nc[j];
+                g->bdn[key] = -1;
                 if (idn[i] >= 0) {
                     it[nit++] = (eng_item_t){idn[i], j, ci, ci + 1, tot, 0, ncb, 0};
-                    bdn[key] = tot;
-                    tot += (long long) sh[idn[i]]->cart_size * ncb;
+                    g->bdn[key] = tot;
+                    tot += (long long) g->ncd[i] * ncb;
                 }
                 it[nit++] = (eng_item_t){iup[i], j, ci, ci + 1, tot, 0, ncb, 0};
-                bup[key] = tot;
-                tot += (long long) sh[iup[i]]->cart_size * ncb;
+                g->bup[key] = tot;
+                tot += (long long) g->ncu[i] * ncb;
             }
-    double *buf = (double *) malloc(sizeof(double) * 4 * (tot > 0 ? tot : 1));
+    g->tot = tot;
+    const int eflags = ((flags & LIBGRPP_BATCH_RECORD_LEVELS) ? ENG_F_LEVELS : 0) |
+                       ((flags & LIBGRPP_BATCH_GRAPH) ? ENG_F_GRAPH : 0);
     eng_plan_t *eng = NULL;
-    int rc = eng_plan_create(next, sh, ncenters, cs, nidx, cidx, nit, it, tot, tot, ENG_ALL, 0, &eng);
-    if (!rc) rc = eng_plan_execute(eng, NULL);
-    if (!rc) rc = eng_plan_download(eng, buf);
-    if (!rc && stats) eng_plan_stats(eng, stats);
-    eng_plan_destroy(eng);
-
-    /* atom of each shell: the centre at its position */
-    int *atom = (int *) malloc(sizeof(int) * (nshells + 1));
-    for (int i = 0; i < nshells; i++) {
-        atom[i] = -1;
-        for (int c = 0; c < ncenters; c++)
-            if (points_equal(shells[i]->origin, xyz + 3 * c)) { atom[i] = c; break; }
-    }
-    if (!rc) {
-        int maxnc = 0;
-        for (int i = 0; i < nshells; i++) if (shells[i]->cart_size > maxnc) maxnc = shells[i]->cart_size;
-        double *bra = (double *) malloc(sizeof(double) * 12 * maxnc * maxnc);
-        double *ket = (double *) malloc(sizeof(double) * 12 * maxnc * maxnc);
-        double *tu = (double *) malloc(sizeof(double) * 4 * (maxnc + 20) * maxnc * 2);
-        double *td = (double *) malloc(sizeof(double) * 4 * (maxnc + 20) * maxnc * 2);
-        for (int i = 0; i < nshells; i++)
-            for (int j = 0; j < nshells; j++)
-                for (int ci = 0; ci < nidx; ci++) {
-                    int c = cidx[ci];
-                    const double *C = xyz + 3 * c;
-                    int AC = points_equal(shells[i]->origin, C), BC = points_equal(shells[j]->origin, C);
-                    if (AC && BC) continue;
-                    int nA = shells[i]->cart_size, nB = shells[j]->cart_size;
-                    long long kij = ((long long) i * nshells + j) * nidx + ci, kji = ((long long) j * nshells + i) * nidx + ci;
-                    if (!AC)
-                        bra_derivative(shells[i]->L, nB, bdn[kij] >= 0 ? buf + bdn[kij] : NULL, tot, buf + bup[kij], tot, bra);
-                    if (!BC) {
-                        /* <i|U|j+-> = s <j+-|U|i>^T, then differentiate as a bra over j */
-                        int ncu = sh[iup[j]]->cart_size, ncd = idn[j] >= 0 ? sh[idn[j]]->cart_size : 0;
-                        for (int op = 0; op < 4; op++) {
-                            double s = op == 0 ? 1.0 : -1.0;
-                            for (int a = 0; a < ncu; a++)
-                                for (int b = 0; b < nA; b++) tu[op * ncu * nA + a * nA + b] = buf[op * tot + bup[kji] + a * nA + b];
-                            for (int a = 0; a < ncd; a++)
-                                for (int b = 0; b < nA; b++) td[op * ncd * nA + a * nA + b] = buf[op * tot + bdn[kji] + a * nA + b];
-                            (void) s;
-                        }
-                        double *kt = (double *) malloc(sizeof(double) * 12 * nA * nB);
-                        bra_derivative(shells[j]->L, nA, ncd ? td : NULL, (long long) ncd * nA, tu, (long long) ncu * nA, kt);
-                        for (int op = 0; op < 4; op++) {
-                            double s = op == 0 ? 1.0 : -1.0;
-                            for (int k = 0; k < 3; k++)
-                                for (int b = 0; b < nB; b++)
-                                    for (int a = 0; a < nA; a++)
-                                        ket[((op * 3 + k) * nA + a) * nB + b] = s * kt[((op * 3 + k) * nB + b) * nA + a];
-                        }
-                        free(kt);
-                    }
-                    for (int op = 0; op < 4; op++)
-                        for (int k = 0; k < 3; k++) {
-                            const double *br = bra + (op * 3 + k) * nA * nB, *kt = ket + (op * 3 + k) * nA * nB;
-                            double *gA = atom[i] >= 0 ? out + ((long long) (op * 3 * ncenters + 3 * atom[i] + k)) * n2 : NULL;
-                            double *gB = atom[j] >= 0 ? out + ((long long) (op * 3 * ncenters + 3 * atom[j] + k)) * n2 : NULL;
-                            double *gC = out + ((long long) (op * 3 * ncenters + 3 * c + k)) * n2;
-                            for (int a = 0; a < nA; a++)
-                                for (int b = 0; b < nB; b++) {
-                                    long long e = (long long) (off[i] + a) * dim + off[j] + b;
-                                    if (!AC) {
-                                        if (gA) gA[e] += br[a * nB + b];
-                                        gC[e] -= br[a * nB + b];
-                                    }
-                                    if (!BC) {
-                                        if (gB) gB[e] += kt[a * nB + b];
-                                        gC[e] -= kt[a * nB + b];
-                                    }
-                                }
-                        }
-                }
-        free(bra);
-        free(ket);
-        free(tu);
-        free(td);
-    }
+    int rc = eng_plan_create(next, sh, ncenters, cs, nidx, g->cidx, nit, it, tot, tot, ENG_ALL, eflags, &eng);
+    /* the engine holds its own copies of the shells */
     for (int i = nshells; i < next; i++) libgrpp_delete_shell(sh[i]);
     free(sh);
     free(idn);
     free(iup);
-    free(cidx);
     free(cs);
     free(it);
-    free(bup);
-    free(bdn);
+    if (rc) {
+        grad_host_free(g);
+        return rc;
+    }
+    libgrpp_batch_plan_t *p = (libgrpp_batch_plan_t *) calloc(1, sizeof(*p));
+    p->eng = eng;
+    p->dim = dim;
+    p->flags = flags;
+    p->out_len = tot;
+    p->grad = g;
+    *plan = p;
+    return 0;
+}
+
+long long libgrpp_batch_plan_gradient_len(const libgrpp_batch_plan_t *p)
+{
+    if (!p->grad) return 0;
+    return 12LL * p->grad->ncenters * p->dim * p->dim;
+}
+
+int libgrpp_batch_plan_download_gradients(libgrpp_batch_plan_t *p, double *out)
+{
+    const grad_host_t *g = p->grad;
+    if (!g) {
+        eng_set_error("download_gradients: the plan was not created by libgrpp_batch_gradients_plan_create");
+        return 2;
+    }
+    const int nshells = g->nshells, ncenters = g->ncenters, nidx = g->nidx, dim = p->dim;
+    const long long n2 = (long long) dim * dim, tot = g->tot;
+    double *buf = (double *) malloc(sizeof(double) * 4 * (tot > 0 ? tot : 1));
+    int rc = eng_plan_download(p->eng, buf);
+    if (rc) {
+        free(buf);
+        return rc;
+    }
+    memset(out, 0, sizeof(double) * 12 * ncenters * n2);
+    int maxnc = 1, maxnu = 1;
+    for (int i = 0; i < nshells; i++) {
+        if (g->nc[i] > maxnc) maxnc = g->nc[i];
+        if (g->ncu[i] > maxnu) maxnu = g->ncu[i];
+    }
+    double *bra = (double *) malloc(sizeof(double) * 12 * maxnc * maxnc);
+    double *ket = (double *) malloc(sizeof(double) * 12 * maxnc * maxnc);
+    double *kt = (double *) malloc(sizeof(double) * 12 * maxnc * maxnc);
+    double *tu = (double *) malloc(sizeof(double) * 4 * maxnu * maxnc);
+    double *td = (double *) malloc(sizeof(double) * 4 * maxnu * maxnc);
+    for (int i = 0; i < nshells; i++)
+        for (int j = 0; j < nshells; j++)
+            for (int ci = 0; ci < nidx; ci++) {
+                const int c = g->cidx[ci];
+                const double *C = g->xyz + 3 * c;
+                const int AC = points_equal(g->origin + 3 * i, C), BC = points_equal(g->origin + 3 * j, C);
+                if (AC && BC) continue;
+                const int nA = g->nc[i], nB = g->nc[j];
+                const long long kij = ((long long) i * nshells + j) * nidx + ci;
+                const long long kji = ((long long) j * nshells + i) * nidx + ci;
+                if (!AC)
+                    bra_derivative(g->L[i], nB, g->bdn[kij] >= 0 ? buf + g->bdn[kij] : NULL, tot, buf + g->bup[kij], tot,
+                                   bra);
+                if (!BC) {
+                    /* <i|U|j+-> = s <j+-|U|i>^T, then differentiate as a bra over j */
+                    const int ncu = g->ncu[j], ncd = g->ncd[j];
+                    for (int op = 0; op < 4; op++) {
+                        for (int a = 0; a < ncu; a++)
+                            for (int b = 0; b < nA; b++) tu[op * ncu * nA + a * nA + b] = buf[op * tot + g->bup[kji] + a * nA + b];
+                        for (int a = 0; a < ncd; a++)
+                            for (int b = 0; b < nA; b++) td[op * ncd * nA + a * nA + b] = buf[op * tot + g->bdn[kji] + a * nA + b];
+                    }
+                    bra_derivative(g->L[j], nA, ncd ? td : NULL, (long long) ncd * nA, tu, (long long) ncu * nA, kt);
+                    for (int op = 0; op < 4; op++) {
+                        const double s = op == 0 ? 1.0 : -1.0;
+                        for (int k = 0; k < 3; k++)
+                            for (int b = 0; b < nB; b++)
+                                for (int a = 0; a < nA; a++)
+                                    ket[((op * 3 + k) * nA + a) * nB + b] = s * kt[((op * 3 + k) * nB + b) * nA + a];
+                    }
+                }
+                for (int op = 0; op < 4; op++)
+                    for (int k = 0; k < 3; k++) {
+                        const double *br = bra + (op * 3 + k) * nA * nB, *kk = ket + (op * 3 + k) * nA * nB;
+                        double *gA = g->atom[i] >= 0 ? out + ((long long) (op * 3 * ncenters + 3 * g->atom[i] + k)) * n2 : NULL;
+                        double *gB = g->atom[j] >= 0 ? out + ((long long) (op * 3 * ncenters + 3 * g->atom[j] + k)) * n2 : NULL;
+                        double *gC = out + ((long long) (op * 3 * ncenters + 3 * c + k)) * n2;
+                        for (int a = 0; a < nA; a++)
+                            for (int b = 0; b < nB; b++) {
+                                const long long e = (long long) (g->off[i] + a) * dim + g->off[j] + b;
+                                if (!AC) {
+                                    if (gA) gA[e] += br[a * nB + b];
+                                    gC[e] -= br[a * nB + b];
+                                }
+                                if (!BC) {
+                                    if (gB) gB[e] += kk[a * nB + b];
+                                    gC[e] -= kk[a * nB + b];
+                                }
+                            }
+                    }
+            }
+    free(bra);
+    free(ket);
+    free(kt);
+    free(tu);
+    free(td);
     free(buf);
-    free(atom);
-    free(off);
+    return 0;
+}
+
+int libgrpp_batch_full_grpp_gradients(int nshells, libgrpp_shell_t *const *shells, int ncenters, const double *xyz,
+                                      libgrpp_grpp_t *const *grpps, int flags, double *out,
+                                      libgrpp_batch_stats_t *stats)
+{
+    libgrpp_batch_plan_t *p = NULL;
+    int rc = libgrpp_batch_gradients_plan_create(nshells, shells, ncenters, xyz, grpps, flags, &p);
+    if (rc) return rc;
+    rc = libgrpp_batch_plan_execute(p, NULL);
+    if (!rc) rc = libgrpp_batch_plan_download_gradients(p, out);
+    if (!rc && stats) libgrpp_batch_plan_stats(p, stats);
+    libgrpp_batch_plan_destroy(p);
     return rc;
 }

@@ -12,8 +12,10 @@ enum {
     ENG_TYPE1 = 1,      /* local U_L (type-1) */
     ENG_TYPE2 = 2,      /* semilocal AREP channels */
     ENG_SO = 4,         /* spin-orbit channels, each scaled by so_scale */
-    ENG_OC = 8,         /* outercore projectors */
-    ENG_ALL = 15
+    ENG_OC = 8,         /* outercore projectors (both parts) */
+    ENG_ALL = 15,
+    ENG_OC_P1 = 16,     /* outercore part 1 only: U|nlj><nlj| + |nlj><nlj|U */
+    ENG_OC_P2 = 32      /* outercore part 2 only: -|nlj><nlj| (U + U')/2 |n'lj><n'lj| */
 };
 
 typedef struct {
@@ -28,6 +30,7 @@ typedef struct {
     libgrpp_potential_t *const *oc_pot;
     libgrpp_shell_t *const *oc_shell;   /* centred at origin regardless of their own origin field */
     int shell_begin, shell_end;         /* shells this centre acts on (end < 0: all) */
+    int oc_i, oc_j;                     /* part 2 restricted to the (oc_i, oc_j) term (oc_i < 0: all) */
 } eng_center_t;
 
 /* one output block: shells (a, b), centres [cb, ce) of the centre-index list, destination

@@ -82,7 +82,7 @@ struct DCenter {
     int ob, oe;           // outercore entries (global OC index)
     int sb, se;           // shells acted on
     int delta_off;        // offset of the n_oc x n_oc delta block
-    int pad;
+    int oc_i, oc_j;       // part 2 restricted to the (oc_i, oc_j) term; oc_i < 0: every term
 };
 
 struct DPot {
@@ -1525,8 +1525,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                 }
                 __syncthreads();
             }
-            // ---- outercore projectors ----
-            if ((parts & 8) && c.oe > c.ob) {
+            // ---- outercore projectors: part 1 (outercore_integrals.c:147-270) and part 2
+            // (:277-390); a part-2-only job may restrict part 2 to one (oi, oj) term ----
+            const bool oc1 = parts & (8 | 16), oc2 = parts & (8 | 32);
+            const bool oc_pair = (parts & 32) && c.oc_i >= 0;
+            if ((oc1 || oc2) && c.oe > c.ob) {
                 const int no = c.oe - c.ob;
                 for (int oi = 0; oi < no; oi++) {
                     const OcEntry e1 = J.oce[c.ob + oi];
@@ -1545,25 +1548,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                         const int e = tid + q * 256;
                         if (e >= nel) break;
                         const int i = r0 + e / ncB, j = e % ncB;
-                        double s = 0.0;
-                        for (int m = 0; m < nm; m++) s += SA[i * nm + m] * UB[j * nm + m] + UA[i * nm + m] * SB[j * nm + m];
-                        acc[0][q] += af * s;
+                        if (oc1) {
+                            double s = 0.0;
+                            for (int m = 0; m < nm; m++) s += SA[i * nm + m] * UB[j * nm + m] + UA[i * nm + m] * SB[j * nm + m];
+                            acc[0][q] += af * s;
 #pragma unroll
-                        for (int eta = 0; eta < 3; eta++) {
-                            const double *Le = Lm + eta * nm * nm;
-                            double so = 0.0;
-                            for (int m1 = 0; m1 < nm; m1++) {
-                                double lsb = 0.0, lub = 0.0;
-                                for (int m2 = 0; m2 < nm; m2++) {
-                                    lsb += Le[m1 * nm + m2] * SB[j * nm + m2];
-                                    lub += Le[m1 * nm + m2] * UB[j * nm + m2];
+                            for (int eta = 0; eta < 3; eta++) {
+                                const double *Le = Lm + eta * nm * nm;
+                                double so = 0.0;
+                                for (int m1 = 0; m1 < nm; m1++) {
+                                    double lsb = 0.0, lub = 0.0;
+                                    for (int m2 = 0; m2 < nm; m2++) {
+                                        lsb += Le[m1 * nm + m2] * SB[j * nm + m2];
+                                        lub += Le[m1 * nm + m2] * UB[j * nm + m2];
+                                    }
+                                    so += UA[i * nm + m1] * lsb + SA[i * nm + m1] * lub;
                                 }
-                                so += UA[i * nm + m1] * lsb + SA[i * nm + m1] * lub;
+                                acc[1 + eta][q] += ef * so;
                             }
-                            acc[1 + eta][q] += ef * so;
                         }
                         // part 2: -P_nlj [(U + U')/2] P_n'lj for OC entries of equal (L, J)
-                        for (int oj = 0; oj < no; oj++) {
+                        for (int oj = 0; oc2 && oj < no; oj++) {
+                            if (oc_pair && (oi != c.oc_i || oj != c.oc_j)) continue;
                             const double d = J.delta[c.delta_off + oi * no + oj];
                             if (d == 0.0) continue;
                             const double *SB2 = J.OCV + J.ocv_tab[(long long) (c.ob + oj) * J.nshell + it.b];
@@ -1906,7 +1912,9 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
             Lmax_pot = std::max(Lmax_pot, pots[k].L);
         }
         d.ob = (int) oce.size();
-        if (parts & ENG_OC)
+        d.oc_i = E.oc_i;
+        d.oc_j = E.oc_j;
+        if (parts & (ENG_OC | ENG_OC_P1 | ENG_OC_P2))
             for (int k = 0; k < E.n_oc; k++) {
                 OcEntry oe;
                 oe.pot = add_pot(E.oc_pot[k], K_OC, 1.0);

@@ -78,6 +78,10 @@ def lib() -> C.CDLL:
         L.libgrpp_batch_plan_destroy.argtypes = [C.c_void_p]
         L.libgrpp_batch_plan_features.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_float)]
         L.libgrpp_batch_plan_set_predictions.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float)]
+        L.libgrpp_batch_gradients_plan_create.argtypes = L.libgrpp_batch_plan_create.argtypes
+        L.libgrpp_batch_plan_gradient_len.restype = C.c_longlong
+        L.libgrpp_batch_plan_gradient_len.argtypes = [C.c_void_p]
+        L.libgrpp_batch_plan_download_gradients.argtypes = [C.c_void_p, dptr]
         L.libgrpp_init()
         _lib = L
     return _lib
@@ -237,6 +241,40 @@ class MoleculesPlan(_LevelHooks):
             d = m.dim
             out.append([flat[k, o:o + d * d].reshape(d, d).copy() for k in range(4)])
             o += d * d
+        return out
+
+    def stats(self):
+        st = Stats()
+        lib().libgrpp_batch_plan_stats(self.h, C.byref(st))
+        return st.as_dict()
+
+    def close(self):
+        if self.h:
+            lib().libgrpp_batch_plan_destroy(self.h)
+            self.h = C.c_void_p()
+        self.o.close()
+
+
+class GradientPlan(_LevelHooks):
+    """Device-resident batched gradients (libgrpp_batch_gradients_plan_create): execute() runs the
+    kernels, download() scatters grad[op, 3*centre+k] as batch_gradients returns it."""
+
+    def __init__(self, sysm: System, flags: int = 0):
+        self.sysm = sysm
+        self.o = _Objects(sysm)
+        self.h = C.c_void_p()
+        rc = lib().libgrpp_batch_gradients_plan_create(len(sysm.shells), self.o.shells, len(sysm.centers),
+                                                       as_dptr(self.o.xyz), self.o.grpps, flags, C.byref(self.h))
+        _check(rc, "libgrpp_batch_gradients_plan_create")
+
+    def execute(self, stream: int = 0):
+        _check(lib().libgrpp_batch_plan_execute(self.h, C.c_void_p(stream)), "libgrpp_batch_plan_execute")
+
+    def download(self) -> np.ndarray:
+        dim, na = self.sysm.dim, len(self.sysm.centers)
+        out = np.zeros((4, 3 * na, dim, dim))
+        assert lib().libgrpp_batch_plan_gradient_len(self.h) == out.size
+        _check(lib().libgrpp_batch_plan_download_gradients(self.h, as_dptr(out)), "libgrpp_batch_plan_download_gradients")
         return out
 
     def stats(self):

@@ -140,13 +140,17 @@ _PER_CALL = {
                    full="orc_full_grpp", grad="orc_full_grpp_gradient"),
     "reference": dict(type1="libgrpp_type1_integrals", type2="libgrpp_type2_integrals",
                       so="libgrpp_spin_orbit_integrals", oc="libgrpp_outercore_potential_integrals",
-                      full="libgrpp_full_grpp_integrals", grad="libgrpp_full_grpp_integrals_gradient"),
+                      full="libgrpp_full_grpp_integrals", grad="libgrpp_full_grpp_integrals_gradient",
+                      oc1="libgrpp_outercore_potential_integrals_part_1",
+                      oc2="libgrpp_outercore_potential_integrals_part_2"),
 }
 
 
 def _per_call(self, what, A, B, Cpos, arg, D=None):
     """Single-task evaluation through the library's own per-call function.  what in
-    type1/type2/so (arg: Potential), oc (arg: list of (Potential, Shell)), full/grad (arg: GRPP)."""
+    type1/type2/so (arg: Potential), oc (arg: list of (Potential, Shell)), full/grad (arg: GRPP);
+    for the compiled reference also oc1 (arg: (Potential, Shell)) and oc2 (arg: two of them), its
+    outercore parts 1 and 2 (outercore_integrals.c:147, :277), which add into zeroed outputs."""
     from libgrpp_amd.abi import PPOT, darr
     fn = getattr(self.lib, _PER_CALL[self.kind][what])
     fn.restype = None
@@ -174,6 +178,17 @@ def _per_call(self, what, A, B, Cpos, arg, D=None):
         for i in range(n):
             self.fac.f["delete_potential"](pots[i])
             self.fac.f["delete_shell"](shs[i])
+        res = np.array([x.reshape(na, nb) for x in ms])
+    elif what in ("oc1", "oc2"):
+        entries = [arg] if what == "oc1" else list(arg)
+        objs = []
+        for pot, sh in entries:
+            objs += [self.fac.potential(pot), self.fac.shell(sh)]
+        ms = [np.zeros(na * nb) for _ in range(4)]
+        fn(a, b, Cc, *objs, *[as_dptr(x) for x in ms])
+        for k in range(0, len(objs), 2):
+            self.fac.f["delete_potential"](objs[k])
+            self.fac.f["delete_shell"](objs[k + 1])
         res = np.array([x.reshape(na, nb) for x in ms])
     elif what == "full":
         h = self.fac.grpp(arg)

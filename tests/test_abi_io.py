@@ -15,10 +15,10 @@ LIB = os.path.join(ROOT, "libgrpp_amd", "libgrpp_amd.so")
 
 def declared_functions():
     names = set()
-    for h in ("libgrpp.h", "libgrpp_amd.h"):
+    for h in ("libgrpp.h", "libgrpp_amd.h", "libgrpp_fortran.h"):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        for m in re.finditer(r"\b(libgrpp_\w+)\s*\(", text):
+        for m in re.finditer(r"\b(libgrpp_\w+|evaluate_\w+_|radial_gto_\w+_)\s*\(", text):
             names.add(m.group(1))
     return sorted(names)
 
