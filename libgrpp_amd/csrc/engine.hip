@@ -294,8 +294,8 @@ __global__ __launch_bounds__(256) void k_tab_wU(const int *__restrict__ pots, De
     double v = 0.0;
     for (int k = 0; k < P.nterm; k++)
         v += J.tcoef[P.toff + k] * pow(r, J.tpow[P.toff + k] - 2) * exp(-J.tzeta[P.toff + k] * r2);
-    // stored as sign(u) sqrt|u|: the product stages sqrt|u| on both operands and the sign on one,
-    // so that entry (a, b) rounds the same whichever side of a tile each row sits on, and the
+    // stored as sign(u) sqrt|u|: the product stages r^n sqrt|u| on both operands and the sign on
+    // one, so that entry (a, b) rounds the same whichever side of a tile each row sits on, and the
     // results do not depend on the order of the rows
     const double u = T.w[i] * v;
     wU[(size_t) blockIdx.y * NPTS + i] = copysign(sqrt(fabs(u)), u);
@@ -690,6 +690,7 @@ __global__ __launch_bounds__(256) void k_radial(const TileItem *__restrict__ til
 {
     __shared__ double As[RK][RT + 2];
     __shared__ double Bs[RK][RT + 2];
+    __shared__ double Rw[LMAX_SHELL + 1][RK], Rs[LMAX_SHELL + 1][RK];   // r^n sqrt|wU|, r^n sign(wU) sqrt|wU|
     __shared__ int sfa[RT], sna[RT], sfb[RT], snb[RT];
     const TileItem t = tiles[blockIdx.x];
     const QJob jb = jobs[t.job];
@@ -742,12 +743,20 @@ __global__ __launch_bounds__(256) void k_radial(const TileItem *__restrict__ til
 #pragma unroll
             for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
         for (int k0 = kb; k0 < ke; k0 += RK) {
+            // the K step's point weights once per power n, then one product per staged element
+            for (int e = tid; e < (LMAX_SHELL + 1) * RK; e += 256) {
+                const int n = e / RK, kk = e % RK, k = k0 + kk;
+                const double w = wp[k], rp = rpow[n * NPTS + k];
+                Rw[n][kk] = rp * fabs(w);
+                Rs[n][kk] = rp * w;
+            }
+            __syncthreads();
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 const int e = tid + 256 * q, r = e >> 5, kk = e & 31, k = k0 + kk;
                 const int fa = sfa[r], fb = sfb[r];
-                As[kk][r] = (fa >= 0) ? F[(size_t) fa * NPTS + k] * rpow[sna[r] * NPTS + k] * fabs(wp[k]) : 0.0;
-                Bs[kk][r] = (fb >= 0) ? F[(size_t) fb * NPTS + k] * rpow[snb[r] * NPTS + k] * wp[k] : 0.0;
+                As[kk][r] = (fa >= 0) ? F[(size_t) fa * NPTS + k] * Rw[sna[r]][kk] : 0.0;
+                Bs[kk][r] = (fb >= 0) ? F[(size_t) fb * NPTS + k] * Rs[snb[r]][kk] : 0.0;
             }
             __syncthreads();
 #pragma unroll 8
@@ -832,6 +841,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 {
     __shared__ double As[RK][RT + 2];
     __shared__ double Bs[RK][RT + 2];
+    __shared__ double Rw[LMAX_SHELL + 1][RK], Rs[LMAX_SHELL + 1][RK];   // r^n sqrt|wU|, r^n sign(wU) sqrt|wU|
     __shared__ int sfa[RT], sna[RT], sfb[RT], snb[RT];
     const TileItem t = tiles[blockIdx.x];
     const QJob jb = jobs[t.job];
@@ -888,12 +898,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 #pragma unroll
             for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
         for (int k0 = kb; k0 < ke; k0 += RK) {
+            // the K step's point weights once per power n, then one product per staged element
+            for (int e = tid; e < (LMAX_SHELL + 1) * RK; e += 256) {
+                const int n = e / RK, kk = e % RK, k = k0 + kk;
+                const double w = wp[k], rp = rpow[n * NPTS + k];
+                Rw[n][kk] = rp * fabs(w);
+                Rs[n][kk] = rp * w;
+            }
+            __syncthreads();
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 const int e = tid + 256 * q, r = e >> 5, kk = e & 31, k = k0 + kk;
                 const int fa = sfa[r], fb = sfb[r];
-                As[kk][r] = (fa >= 0) ? F[(size_t) fa * NPTS + k] * rpow[sna[r] * NPTS + k] * fabs(wp[k]) : 0.0;
-                Bs[kk][r] = (fb >= 0) ? F[(size_t) fb * NPTS + k] * rpow[snb[r] * NPTS + k] * wp[k] : 0.0;
+                As[kk][r] = (fa >= 0) ? F[(size_t) fa * NPTS + k] * Rw[sna[r]][kk] : 0.0;
+                Bs[kk][r] = (fb >= 0) ? F[(size_t) fb * NPTS + k] * Rs[snb[r]][kk] : 0.0;
             }
             __syncthreads();
 #pragma unroll
