@@ -1408,7 +1408,8 @@ __global__ void k_delta(const int *__restrict__ centers, int n, DevJob J, double
 }
 
 // ================================================================== contraction
-constexpr int CT_EPT = 1;   // output elements per thread (<= 256 per row chunk)
+constexpr int CT_THREADS = 128;   // threads of a k_contract workgroup
+constexpr int CT_EPT = 2;   // output elements per thread (<= CT_EPT * CT_THREADS per row chunk)
 
 __device__ __forceinline__ double int_gauss(int n, double a)
 {
@@ -1429,7 +1430,7 @@ constexpr int TS_MAX = 6144;   // upper bound of the LDS doubles for the (m, i, 
 // One workgroup per (shell pair, centre) (PartItem): the centre's contribution to the four blocks
 // goes to its own slice of the partial buffer; k_contract_reduce adds the slices of a pair in
 // centre order.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_contract(const PartItem *__restrict__ partlist, const WorkItem *__restrict__ items,
+__global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_contract(const PartItem *__restrict__ partlist, const WorkItem *__restrict__ items,
                                                   DevJob J, DevTables T, double *__restrict__ partial, int parts,
                                                   int nm_max, int qs_len, int ts_len)
 {
@@ -1445,7 +1446,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     const int tid = threadIdx.x;
     const double FOURPI2 = 16.0 * M_PI * M_PI;
     // rows of A per pass: the outputs fit CT_EPT per thread and T[m][i][k2] fits TS_MAX
-    const int rows_per_chunk = max(1, min((CT_EPT * 256) / ncB, ts_len / (nm_max * ncB)));
+    const int rows_per_chunk = max(1, min((CT_EPT * CT_THREADS) / ncB, ts_len / (nm_max * ncB)));
     for (int r0 = 0; r0 < ncA; r0 += rows_per_chunk) {
         const int r1 = min(ncA, r0 + rows_per_chunk), nr = r1 - r0, nel = nr * ncB;
         double acc[4][CT_EPT];
@@ -1511,13 +1512,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                     const int k2 = e % KB, mi = e / KB, i = mi % nr, m = mi / nr;
                     const double *pa = PA + ((long long) (r0 + i) * nm + m) * KA;
                     double sacc = 0.0;
+#pragma unroll 4
                     for (int k1 = 0; k1 < KA; k1++) sacc += pa[k1] * Qs[k1 * KB + k2];
                     Ts[e] = sacc;
                 }
                 __syncthreads();
 #pragma unroll
                 for (int q = 0; q < CT_EPT; q++) {
-                    const int e = tid + q * 256;
+                    const int e = tid + q * CT_THREADS;
                     if (e >= nel) break;
                     const int i = e / ncB, j = e % ncB;
                     if (!is_so) {
@@ -1525,6 +1527,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                         for (int m = 0; m < nm; m++) {
                             const double *tr = Ts + (m * nr + i) * KB;
                             const double *pb = PB + ((long long) j * nm + m) * KB;
+#pragma unroll 4
                             for (int k2 = 0; k2 < KB; k2++) sacc += tr[k2] * pb[k2];
                         }
                         acc[0][q] += fac * sacc;
@@ -1535,6 +1538,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                             for (int m = 0; m < nm; m++) {
                                 const double *tr = Ts + (m * nr + i) * KB;
                                 const double *pl = PLB + (((long long) eta * ncB + j) * nm + m) * KB;
+#pragma unroll 4
                                 for (int k2 = 0; k2 < KB; k2++) sacc += tr[k2] * pl[k2];
                             }
                             acc[1 + eta][q] += fac * sacc;
@@ -1563,7 +1567,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                     const double *Lm = T.lmat + T.lmat_off[L];
 #pragma unroll
                     for (int q = 0; q < CT_EPT; q++) {
-                        const int e = tid + q * 256;
+                        const int e = tid + q * CT_THREADS;
                         if (e >= nel) break;
                         const int i = r0 + e / ncB, j = e % ncB;
                         if (oc1) {
@@ -1613,7 +1617,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         // ---- this centre's slice of the partial blocks ----
 #pragma unroll
         for (int q = 0; q < CT_EPT; q++) {
-            const int e = tid + q * 256;
+            const int e = tid + q * CT_THREADS;
             if (e >= nel) break;
             const int i = r0 + e / ncB, j = e % ncB;
 #pragma unroll
@@ -2410,7 +2414,7 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
     HIPCHK(hipStreamWaitEvent(st, P->evs[2], 0));   // join: the contraction reads projectors and type-1 slices
     HIPCHK(hipEventRecord(P->ev[6], st));
     if (P->n_parts > 0)
-        hipLaunchKernelGGL(k_contract, dim3(P->n_parts), dim3(256), (size_t) (P->ct_qs + P->ct_ts) * sizeof(double),
+        hipLaunchKernelGGL(k_contract, dim3(P->n_parts), dim3(CT_THREADS), (size_t) (P->ct_qs + P->ct_ts) * sizeof(double),
                            st, P->d_parts, P->d_items, P->J, P->T, P->d_partial, P->parts, P->nm_max, P->ct_qs,
                            P->ct_ts);
     if (P->n_items > 0)   // also writes the zero blocks of items without centres
