@@ -258,6 +258,7 @@ __global__ __launch_bounds__(256) void k_tab_F(const TabItem *__restrict__ items
         if (cf == 0.0) continue;
         const double a = J.alpha[s.poff + k];
         const double e = cf * exp(-a * d2);
+        if (e == 0.0) continue;   // the primitive vanishes at this point (K_l <= 1): adds exactly 0
         const double x = 2.0 * a * CA * r;
         if (x > 1e-7 && x <= 16.0) {
             const int i0 = (int) (rint(x * 100) + 1e-5);
@@ -1428,8 +1429,8 @@ __device__ __forceinline__ double int_gauss(int n, double a)
 constexpr int TS_MAX = 6144;   // upper bound of the LDS doubles for the (m, i, k2) intermediate
 
 // One workgroup per (shell pair, centre) (PartItem): the centre's contribution to the four blocks
-// goes to its own slice of the partial buffer; k_contract_reduce adds the slices of a pair in
-// centre order.
+// goes to its own slice of the contraction's partial buffer; k_contract_reduce adds, in centre
+// order, each centre's type-1 slice plus its contraction slice.
 __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_contract(const PartItem *__restrict__ partlist, const WorkItem *__restrict__ items,
                                                   DevJob J, DevTables T, double *__restrict__ partial, int parts,
                                                   int nm_max, int qs_len, int ts_len)
@@ -1621,18 +1622,19 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
             if (e >= nel) break;
             const int i = r0 + e / ncB, j = e % ncB;
 #pragma unroll
-            for (int op = 0; op < 4; op++) partial[pt.off + ((long long) op * ncA + i) * ncB + j] += acc[op][q];
+            for (int op = 0; op < 4; op++) partial[pt.off + ((long long) op * ncA + i) * ncB + j] = acc[op][q];
         }
         __syncthreads();
     }
 }
 
-// out = sum over the item's centres, in centre order, of the partial slices;
+// out = sum over the item's centres, in centre order, of (type-1 slice + contraction slice);
 // the (B, A) block of a mirrored item is the (anti)symmetric image
 __global__ __launch_bounds__(256) void k_contract_reduce(const WorkItem *__restrict__ items,
                                                          const int *__restrict__ item_part0,
                                                          const PartItem *__restrict__ partlist, DevJob J,
-                                                         const double *__restrict__ partial, double *__restrict__ out)
+                                                         const double *__restrict__ partial_t1,
+                                                         const double *__restrict__ partial_ct, double *__restrict__ out)
 {
     const WorkItem it = items[blockIdx.x];
     const int ncA = dev_ncart(J.sh[it.a].L), ncB = dev_ncart(J.sh[it.b].L), nel = ncA * ncB;
@@ -1640,7 +1642,10 @@ __global__ __launch_bounds__(256) void k_contract_reduce(const WorkItem *__restr
     for (int e = threadIdx.x; e < 4 * nel; e += blockDim.x) {
         const int op = e / nel, ij = e % nel, i = ij / ncB, j = ij % ncB;
         double s = 0.0;
-        for (int k = 0; k < np; k++) s += partial[partlist[p0 + k].off + e];
+        for (int k = 0; k < np; k++) {
+            const long long o = partlist[p0 + k].off + e;
+            s += partial_t1[o] + partial_ct[o];   // the centre's type-1 slice plus its contraction slice
+        }
         out[op * J.op_stride + it.off + (long long) i * it.stride + j] = s;
         if (it.mirror) out[op * J.op_stride + it.moff + (long long) j * it.stride + i] = (op == 0 ? 1.0 : -1.0) * s;
     }
@@ -1772,7 +1777,8 @@ struct eng_plan {
     int n_items = 0;
     PartItem *d_parts = nullptr;     // (item, centre) workgroups of the contraction
     int *d_item_part0 = nullptr;     // first part of each item
-    double *d_partial = nullptr;     // per-part partial blocks
+    double *d_partial = nullptr;     // per-part type-1 blocks (k_type1)
+    double *d_partial_ct = nullptr;  // per-part semilocal, spin-orbit and outercore blocks (k_contract)
     int n_parts = 0;
     unsigned long long *d_counters = nullptr;
     hipEvent_t ev[8] = {};   // execution phases on the caller's stream: see eng_plan_execute
@@ -2220,6 +2226,7 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     rc |= dput(P, &P->d_parts, partlist);
     rc |= dput(P, &P->d_item_part0, item_part0);
     rc |= dalloc(P, &P->d_partial, (size_t) partial_len);
+    rc |= dalloc(P, &P->d_partial_ct, (size_t) partial_len);
     rc |= dalloc(P, &P->d_F, (size_t) frows * NPTS);
     rc |= dalloc(P, &P->d_wU, (size_t) wpots.size() * NPTS);
     rc |= dalloc(P, &P->d_Q, (size_t) qlen);
@@ -2355,12 +2362,13 @@ __global__ void k_zero_u64(unsigned long long *p, int n)
 // It makes no allocation, memset or synchronisation, so it can be captured into a graph.
 static int enqueue_build(eng_plan_t *P, hipStream_t st)
 {
-    // Two streams of default priority.  The caller's stream runs the radial chain (tables,
-    // screening, radial product, canonical splits); the side stream runs what does not depend on
-    // it (grid tables, projectors, outercore vectors, type-1), forked after the counters are cleared;
-    // the radial product waits for the grid tables (ev 5) and the contraction joins the rest.
-    // Events: caller 0 start = 1 | screening | 2 | radial | 3 | canonical | 4 ... join ... 6 |
-    // contraction | 7; side s0 | tables | 5 | projectors, outercore | s1 | type-1 | s2.
+    // Two streams of default priority.  The caller's stream runs the radial chain (screening,
+    // radial product, canonical splits) and the contraction; the side stream runs what does not
+    // depend on it (grid tables, projectors, outercore vectors, type-1), forked after the counters
+    // are cleared.  The radial product waits for the grid tables (ev 5), the contraction for the
+    // projectors (s1), the final reduction for the type-1 slices (s2).
+    // Events: caller 0 start = 1 | screening | 2 | radial | 3 | canonical | 4 ... 6 | contraction,
+    // reduction | 7; side s0 | tables | 5 | projectors, outercore | s1 | type-1 | s2.
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, P->d_counters, 8);
     HIPCHK(hipEventRecord(P->ev[0], st));
     HIPCHK(hipStreamWaitEvent(P->side, P->ev[0], 0));
@@ -2411,15 +2419,16 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
         hipLaunchKernelGGL(k_canon, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->J, P->d_Q,
                            P->d_Qc);
     HIPCHK(hipEventRecord(P->ev[4], st));
-    HIPCHK(hipStreamWaitEvent(st, P->evs[2], 0));   // join: the contraction reads projectors and type-1 slices
+    HIPCHK(hipStreamWaitEvent(st, P->evs[1], 0));   // the contraction reads projectors and outercore vectors
     HIPCHK(hipEventRecord(P->ev[6], st));
     if (P->n_parts > 0)
         hipLaunchKernelGGL(k_contract, dim3(P->n_parts), dim3(CT_THREADS), (size_t) (P->ct_qs + P->ct_ts) * sizeof(double),
-                           st, P->d_parts, P->d_items, P->J, P->T, P->d_partial, P->parts, P->nm_max, P->ct_qs,
+                           st, P->d_parts, P->d_items, P->J, P->T, P->d_partial_ct, P->parts, P->nm_max, P->ct_qs,
                            P->ct_ts);
+    HIPCHK(hipStreamWaitEvent(st, P->evs[2], 0));   // join: the reduction reads the type-1 slices
     if (P->n_items > 0)   // also writes the zero blocks of items without centres
         hipLaunchKernelGGL(k_contract_reduce, dim3(P->n_items), dim3(256), 0, st, P->d_items, P->d_item_part0,
-                           P->d_parts, P->J, P->d_partial, P->d_out);
+                           P->d_parts, P->J, P->d_partial, P->d_partial_ct, P->d_out);
     HIPCHK(hipGetLastError());
     return 0;
 }
