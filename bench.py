@@ -107,6 +107,9 @@ def main():
     ap.add_argument("--workload", default="heavy_cluster")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="time the plain method only")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuning)")
+    ap.add_argument("--tuning", default=None,
+                    help="fixed launch shapes screen_probe,contract_threads,type1_threads instead of autotuning")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device-to-device copy (3 times): the known-size kernel "
@@ -149,8 +152,21 @@ def main():
         wall = time.perf_counter() - t0
         return wall, plan.stats(), {k: float(np.mean(v)) for k, v in times.items()}
 
+    def tune(plan):
+        """the plan's launch shapes picked by timing its own builds (untimed; results unchanged)"""
+        if args.tuning:
+            probe, ct, t1 = (int(x) for x in args.tuning.split(","))
+            plan.set_tuning(probe, ct, t1)
+            return {"screen_probe": probe, "contract_threads": ct, "type1_threads": t1, "fixed": True}
+        if args.no_tune:
+            return None
+        plan.execute()
+        reps = 2 if plan.stats()["device_seconds"] < 0.2 else 1
+        return plan.autotune(repeats=reps)
+
     # plain adaptive method
     plan = LG.make_plan(sysm, 0)
+    tuning_plain = tune(plan)
     wall_plain, st_plain, mean_plain = timed(plan)
     plain_out = plan.download()
     plan.close()
@@ -169,6 +185,7 @@ def main():
         LG.apply(plan, net, bf16=True)
         torch.cuda.synchronize()
         t_pred = time.perf_counter() - t0
+        tuning = tune(plan)
         wall, st, mean = timed(plan)
         out = plan.download()
         plan.close()
@@ -184,7 +201,8 @@ def main():
                         "radial_ms_plain": 1e3 * mean_plain["radial_seconds"],
                         "radial_ms_learned": 1e3 * mean["radial_seconds"],
                         "max_abs_diff_vs_plain": diff, "training_rows": int(X.shape[0]),
-                        "training_loss": loss, "training_s": t_train, "inference_and_reorder_ms": 1e3 * t_pred}
+                        "training_loss": loss, "training_s": t_train, "inference_and_reorder_ms": 1e3 * t_pred,
+                        "autotune_plain": tuning_plain, "autotune_learned": tuning}
 
     ntask = st["n_tasks"]
     nint = st["n_integrals"]
@@ -226,6 +244,7 @@ def main():
     }
     if learned_info:
         line["learned_grid"] = learned_info
+    line["autotune"] = learned_info["autotune_learned"] if learned_info else tuning_plain
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(sysm, args.cpu_budget)
     print(json.dumps(line))

@@ -99,6 +99,22 @@ int libgrpp_batch_molecules_plan_create(int nmol, const int *shell_off, const in
 long long libgrpp_batch_plan_output_len(const libgrpp_batch_plan_t *plan);   /* doubles per operator */
 int libgrpp_batch_plan_download_flat(libgrpp_batch_plan_t *plan, double *out);   /* 4 * output_len doubles */
 int libgrpp_batch_plan_dim(const libgrpp_batch_plan_t *plan);
+/*
+ * Autotuner: launch shapes of a plan that change no result (bitwise), picked by timing the plan's
+ * own build: the screening probe width (per-phase time of the screening), the contraction's and
+ * the type-1 workgroup sizes (contraction phase, whole build).  Each candidate runs `repeats`
+ * eager builds and keeps its fastest; the choice stays with the plan (a captured graph is
+ * recaptured at the next execute).
+ */
+typedef struct {
+    int screen_probe;         /* 4, 8 or 16 triples per entry in the screening probe */
+    int contract_threads;     /* 64, 128 or 256 */
+    int type1_threads;        /* 64, 128 or 256 */
+    double seconds_before;    /* fastest build with the shapes in place before tuning */
+    double seconds_after;     /* fastest build with the chosen shapes */
+} libgrpp_batch_tuning_t;
+int libgrpp_batch_plan_autotune(libgrpp_batch_plan_t *plan, int repeats, libgrpp_batch_tuning_t *chosen);
+int libgrpp_batch_plan_set_tuning(libgrpp_batch_plan_t *plan, const libgrpp_batch_tuning_t *tuning);
 int libgrpp_batch_plan_stats(const libgrpp_batch_plan_t *plan, libgrpp_batch_stats_t *stats);
 /*
  * Gradient plan: the inputs of libgrpp_batch_full_grpp_gradients packed once.  execute evaluates

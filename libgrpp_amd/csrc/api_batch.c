@@ -487,6 +487,68 @@ int libgrpp_batch_plan_download(libgrpp_batch_plan_t *p, double *arep, double *s
 }
 
 int libgrpp_batch_plan_dim(const libgrpp_batch_plan_t *p) { return p->dim; }
+
+int libgrpp_batch_plan_set_tuning(libgrpp_batch_plan_t *p, const libgrpp_batch_tuning_t *t)
+{
+    eng_tuning_t e = {t->screen_probe, t->contract_threads, t->type1_threads};
+    return eng_plan_set_tuning(p->eng, &e);
+}
+
+/* fastest of `repeats` eager builds: the whole build and the phase named by `which`
+ * (0 whole, 1 screening, 2 contraction) */
+static int time_builds(libgrpp_batch_plan_t *p, int repeats, int which, double *best)
+{
+    *best = 1e300;
+    for (int r = 0; r < repeats; r++) {
+        int rc = eng_plan_execute_eager(p->eng, NULL);
+        if (rc) return rc;
+        libgrpp_batch_stats_t st;
+        eng_plan_stats(p->eng, &st);
+        double v = which == 1 ? st.screen_seconds : which == 2 ? st.contract_seconds : st.device_seconds;
+        if (v < *best) *best = v;
+    }
+    return 0;
+}
+
+int libgrpp_batch_plan_autotune(libgrpp_batch_plan_t *p, int repeats, libgrpp_batch_tuning_t *chosen)
+{
+    if (repeats < 1) repeats = 1;
+    eng_tuning_t t;
+    eng_plan_get_tuning(p->eng, &t);
+    double before = 0.0;
+    int rc = time_builds(p, repeats, 0, &before);
+    if (rc) return rc;
+    /* one parameter at a time, each judged by the phase it acts on */
+    static const int probes[3] = {4, 8, 16}, threads[3] = {64, 128, 256};
+    for (int par = 0; par < 3; par++) {
+        int *field = par == 0 ? &t.screen_probe : par == 1 ? &t.contract_threads : &t.type1_threads;
+        const int *cand = par == 0 ? probes : threads;
+        int best_v = *field;
+        double best_t = 1e300;
+        for (int c = 0; c < 3; c++) {
+            *field = cand[c];
+            if ((rc = eng_plan_set_tuning(p->eng, &t))) return rc;
+            double v;
+            if ((rc = time_builds(p, repeats, par == 0 ? 1 : par == 1 ? 2 : 0, &v))) return rc;
+            if (v < best_t) {
+                best_t = v;
+                best_v = cand[c];
+            }
+        }
+        *field = best_v;
+    }
+    if ((rc = eng_plan_set_tuning(p->eng, &t))) return rc;
+    double after = 0.0;
+    if ((rc = time_builds(p, repeats, 0, &after))) return rc;
+    if (chosen) {
+        chosen->screen_probe = t.screen_probe;
+        chosen->contract_threads = t.contract_threads;
+        chosen->type1_threads = t.type1_threads;
+        chosen->seconds_before = before;
+        chosen->seconds_after = after;
+    }
+    return 0;
+}
 int libgrpp_batch_plan_stats(const libgrpp_batch_plan_t *p, libgrpp_batch_stats_t *s) { return eng_plan_stats(p->eng, s); }
 int libgrpp_batch_plan_features(libgrpp_batch_plan_t *p, int *n, float *f) { return eng_plan_features(p->eng, n, f); }
 int libgrpp_batch_plan_row_levels(libgrpp_batch_plan_t *p, int *n, float *v) { return eng_plan_row_levels(p->eng, n, v); }

@@ -44,6 +44,13 @@ typedef struct {
 
 typedef struct eng_plan eng_plan_t;
 
+/* launch shapes of a plan that do not change any result (the autotuner's choices) */
+typedef struct {
+    int screen_probe;       /* triples per entry in the screening probe: 4, 8 or 16 */
+    int contract_threads;   /* k_contract workgroup: 64, 128 or 256 threads */
+    int type1_threads;      /* k_type1 workgroup: 64, 128 or 256 threads */
+} eng_tuning_t;
+
 /* flags */
 enum { ENG_F_LEARNED = 1, ENG_F_STATS = 2, ENG_F_LEVELS = 4, ENG_F_GRAPH = 8 };
 #define ENG_NFEAT 12   /* level-predictor features per radial row */
@@ -59,6 +66,9 @@ int eng_plan_stats(const eng_plan_t *plan, libgrpp_batch_stats_t *stats);
 int eng_plan_features(eng_plan_t *plan, int *n, float *features);
 int eng_plan_set_predictions(eng_plan_t *plan, int n, const float *pred);
 int eng_plan_row_levels(eng_plan_t *plan, int *n, float *levels);
+int eng_plan_get_tuning(const eng_plan_t *plan, eng_tuning_t *t);
+int eng_plan_set_tuning(eng_plan_t *plan, const eng_tuning_t *t);   /* invalidates a captured graph */
+int eng_plan_execute_eager(eng_plan_t *plan, void *stream);   /* never the graph: phases are timed */
 
 /* convenience: create + execute + download + destroy */
 int eng_run(int nshell, libgrpp_shell_t *const *shells, int ncenter, const eng_center_t *centers, int nidx,

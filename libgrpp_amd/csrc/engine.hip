@@ -435,7 +435,8 @@ __device__ __forceinline__ double screen_value(const DevTables &T, int n, int l1
            exp(-eta * r0 * r0 - aA * dA * dA - aB * dB * dB) * 0.5 * (1.0 + erf(sqrt(p) * r0));
 }
 
-constexpr int SCR_PROBE = 8;   // triples per entry in the screening probe (eight entries per wave)
+// triples per entry in the screening probe (64 / PROBE entries per wave): 4, 8 or 16, picked per
+// plan by the autotuner (the estimates do not depend on it)
 
 // Fixed point of triple q of an entry in the probe order (smallest exponents first: the triples most
 // likely to fail); active is false for triples that vanish
@@ -513,6 +514,7 @@ __device__ double screen_entry_wave(const DevJob &J, const DevTables &T, const D
     return est;
 }
 
+template <int SCR_PROBE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_screen(
     const TileItem *__restrict__ tiles, const QJob *__restrict__ jobs, const RowDesc *__restrict__ rows, DevJob J,
     DevTables T, double *__restrict__ scr, unsigned long long *__restrict__ counters)
@@ -1409,8 +1411,9 @@ __global__ void k_delta(const int *__restrict__ centers, int n, DevJob J, double
 }
 
 // ================================================================== contraction
-constexpr int CT_THREADS = 128;   // threads of a k_contract workgroup
-constexpr int CT_EPT = 2;   // output elements per thread (<= CT_EPT * CT_THREADS per row chunk)
+// k_contract workgroups are CT_THREADS = 64, 128 or 256 threads (picked per plan by the autotuner)
+// with CT_EPT = 256 / CT_THREADS outputs each, so a row chunk holds up to 256 outputs in every case
+// and every output is the same sum
 
 __device__ __forceinline__ double int_gauss(int n, double a)
 {
@@ -1431,10 +1434,12 @@ constexpr int TS_MAX = 6144;   // upper bound of the LDS doubles for the (m, i, 
 // One workgroup per (shell pair, centre) (PartItem): the centre's contribution to the four blocks
 // goes to its own slice of the contraction's partial buffer; k_contract_reduce adds, in centre
 // order, each centre's type-1 slice plus its contraction slice.
+template <int CT_THREADS>
 __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_contract(const PartItem *__restrict__ partlist, const WorkItem *__restrict__ items,
                                                   DevJob J, DevTables T, double *__restrict__ partial, int parts,
                                                   int nm_max, int qs_len, int ts_len)
 {
+    constexpr int CT_EPT = 256 / CT_THREADS;
     // dynamic LDS sized by the plan: Q block (qs_len doubles) + (m, i, k2) intermediate (ts_len)
     extern __shared__ double ct_lds[];
     double *Qs = ct_lds, *Ts = ct_lds + qs_len;
@@ -1781,7 +1786,8 @@ struct eng_plan {
     double *d_partial_ct = nullptr;  // per-part semilocal, spin-orbit and outercore blocks (k_contract)
     int n_parts = 0;
     unsigned long long *d_counters = nullptr;
-    hipEvent_t ev[8] = {};   // execution phases on the caller's stream: see eng_plan_execute
+    eng_tuning_t tune{8, 128, 256};   // launch shapes; none of them changes a result
+    hipEvent_t ev[8] = {};   // execution phases on the caller's stream: see eng_plan_execute   // execution phases on the caller's stream: see eng_plan_execute
     hipStream_t side = nullptr;   // second stream (default priority) for the radial-independent kernels
     hipStream_t cap = nullptr;    // capture stream of a graph plan (default priority)
     hipGraph_t graph = nullptr;
@@ -2353,6 +2359,12 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
 
 static int eager_phase_times(eng_plan_t *P);
 
+typedef void (*contract_fn)(const PartItem *, const WorkItem *, DevJob, DevTables, double *, int, int, int, int);
+static contract_fn contract_kernel(int threads)
+{
+    return threads == 64 ? k_contract<64> : threads == 256 ? k_contract<256> : k_contract<128>;
+}
+
 __global__ void k_zero_u64(unsigned long long *p, int n)
 {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
@@ -2391,13 +2403,16 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
     if (P->n_ocv > 0) hipLaunchKernelGGL(k_ocv, dim3(P->n_ocv), dim3(256), 0, sd, P->d_ocvi, P->J, P->T, P->d_OCV);
     HIPCHK(hipEventRecord(P->evs[1], sd));
     if (P->n_parts > 0)
-        hipLaunchKernelGGL(k_type1, dim3(P->n_parts), dim3(256), (size_t) (P->t1_scratch + P->t1_acc) * sizeof(double),
+        hipLaunchKernelGGL(k_type1, dim3(P->n_parts), dim3(P->tune.type1_threads),
+                           (size_t) (P->t1_scratch + P->t1_acc) * sizeof(double),
                            sd, P->d_parts, P->d_items, P->J, P->T, P->d_partial, P->parts, P->t1_scratch);
     HIPCHK(hipEventRecord(P->evs[2], sd));
     HIPCHK(hipEventRecord(P->ev[1], st));
-    if (P->n_tiles > 0)
-        hipLaunchKernelGGL(k_screen, dim3(4 * P->n_tiles), dim3(64), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->J, P->T,
+    if (P->n_tiles > 0) {
+        auto screen = P->tune.screen_probe == 4 ? k_screen<4> : P->tune.screen_probe == 16 ? k_screen<16> : k_screen<8>;
+        hipLaunchKernelGGL(screen, dim3(4 * P->n_tiles), dim3(64), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->J, P->T,
                            P->d_scr, P->d_counters);
+    }
     HIPCHK(hipStreamWaitEvent(st, P->ev[5], 0));   // grid tables of the side stream
     HIPCHK(hipEventRecord(P->ev[2], st));
     if (P->n_tiles > 0) {
@@ -2422,9 +2437,9 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
     HIPCHK(hipStreamWaitEvent(st, P->evs[1], 0));   // the contraction reads projectors and outercore vectors
     HIPCHK(hipEventRecord(P->ev[6], st));
     if (P->n_parts > 0)
-        hipLaunchKernelGGL(k_contract, dim3(P->n_parts), dim3(CT_THREADS), (size_t) (P->ct_qs + P->ct_ts) * sizeof(double),
-                           st, P->d_parts, P->d_items, P->J, P->T, P->d_partial_ct, P->parts, P->nm_max, P->ct_qs,
-                           P->ct_ts);
+        hipLaunchKernelGGL(contract_kernel(P->tune.contract_threads), dim3(P->n_parts), dim3(P->tune.contract_threads),
+                           (size_t) (P->ct_qs + P->ct_ts) * sizeof(double), st, P->d_parts, P->d_items, P->J, P->T,
+                           P->d_partial_ct, P->parts, P->nm_max, P->ct_qs, P->ct_ts);
     HIPCHK(hipStreamWaitEvent(st, P->evs[2], 0));   // join: the reduction reads the type-1 slices
     if (P->n_items > 0)   // also writes the zero blocks of items without centres
         hipLaunchKernelGGL(k_contract_reduce, dim3(P->n_items), dim3(256), 0, st, P->d_items, P->d_item_part0,
@@ -2441,8 +2456,8 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
     if (P->n_parts > 0) {
         HIPCHK(hipFuncSetAttribute((const void *) k_type1, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (P->t1_scratch + P->t1_acc) * (int) sizeof(double)));
-        HIPCHK(hipFuncSetAttribute((const void *) k_contract, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (P->ct_qs + P->ct_ts) * (int) sizeof(double)));
+        HIPCHK(hipFuncSetAttribute((const void *) contract_kernel(P->tune.contract_threads),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (P->ct_qs + P->ct_ts) * (int) sizeof(double)));
     }
     if (P->flags & ENG_F_GRAPH) {
         // the build is captured once, on a plan-owned stream, and replayed on the caller's stream;
@@ -2565,6 +2580,39 @@ extern "C" int eng_plan_features(eng_plan_t *P, int *n, float *features)
 
 // Per logical row: [max, mean, fraction >= 7] of the stopping levels of its entries over every
 // job of its slot (screened entries count as level 0, not converged as 8), from the last execute.
+extern "C" int eng_plan_get_tuning(const eng_plan_t *P, eng_tuning_t *t)
+{
+    *t = P->tune;
+    return 0;
+}
+
+extern "C" int eng_plan_set_tuning(eng_plan_t *P, const eng_tuning_t *t)
+{
+    auto in3 = [](int v, int a, int b, int c) { return v == a || v == b || v == c; };
+    if (!in3(t->screen_probe, 4, 8, 16) || !in3(t->contract_threads, 64, 128, 256) ||
+        !in3(t->type1_threads, 64, 128, 256)) {
+        eng_set_error("tuning: screen_probe must be 4, 8 or 16 and the thread counts 64, 128 or 256");
+        return 2;
+    }
+    P->tune = *t;
+    if (P->gexec) {   // the captured graph holds the old launch shapes
+        (void) hipGraphExecDestroy(P->gexec);
+        (void) hipGraphDestroy(P->graph);
+        P->gexec = nullptr;
+        P->graph = nullptr;
+    }
+    return 0;
+}
+
+extern "C" int eng_plan_execute_eager(eng_plan_t *P, void *stream)
+{
+    const int f = P->flags;
+    P->flags &= ~ENG_F_GRAPH;
+    const int rc = eng_plan_execute(P, stream);
+    P->flags = f;
+    return rc;
+}
+
 extern "C" int eng_plan_row_levels(eng_plan_t *P, int *n, float *out)
 {
     *n = (int) P->rows_logical.size();

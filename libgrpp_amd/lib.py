@@ -79,6 +79,8 @@ def lib() -> C.CDLL:
         L.libgrpp_batch_plan_features.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_float)]
         L.libgrpp_batch_plan_set_predictions.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float)]
         L.libgrpp_batch_gradients_plan_create.argtypes = L.libgrpp_batch_plan_create.argtypes
+        L.libgrpp_batch_plan_autotune.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.libgrpp_batch_plan_set_tuning.argtypes = [C.c_void_p, C.c_void_p]
         L.libgrpp_batch_plan_gradient_len.restype = C.c_longlong
         L.libgrpp_batch_plan_gradient_len.argtypes = [C.c_void_p]
         L.libgrpp_batch_plan_download_gradients.argtypes = [C.c_void_p, dptr]
@@ -150,8 +152,26 @@ def batch_gradients(sysm: System, flags: int = 0):
     return out, st.as_dict()
 
 
+class Tuning(C.Structure):
+    """libgrpp_batch_tuning_t"""
+    _fields_ = [("screen_probe", C.c_int), ("contract_threads", C.c_int), ("type1_threads", C.c_int),
+                ("seconds_before", C.c_double), ("seconds_after", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class _LevelHooks:
-    """Radial-grid level predictor hooks of a plan handle ``self.h``."""
+    """Radial-grid level predictor hooks and the autotuner of a plan handle ``self.h``."""
+
+    def autotune(self, repeats: int = 2) -> dict:
+        t = Tuning()
+        _check(lib().libgrpp_batch_plan_autotune(self.h, repeats, C.byref(t)), "libgrpp_batch_plan_autotune")
+        return t.as_dict()
+
+    def set_tuning(self, screen_probe: int = 8, contract_threads: int = 128, type1_threads: int = 256):
+        t = Tuning(screen_probe, contract_threads, type1_threads, 0.0, 0.0)
+        _check(lib().libgrpp_batch_plan_set_tuning(self.h, C.byref(t)), "libgrpp_batch_plan_set_tuning")
 
     def features(self) -> np.ndarray:
         n = C.c_int()

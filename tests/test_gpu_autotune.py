@@ -1,0 +1,45 @@
+"""The autotuner changes launch shapes only: every shape it may pick gives bitwise the matrices of
+the default shapes (screening probe width 4/8/16, contraction and type-1 workgroups of 64/128/256
+threads), and a tuned plan, eager or graph-replayed, still does."""
+import numpy as np
+import pytest
+
+from libgrpp_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("workload", ["small_pair", "heavy_cluster"])
+def test_every_launch_shape_gives_the_same_matrices(workload):
+    from libgrpp_amd import lib as G
+    sysm = getattr(configs, workload)()
+    plan = G.Plan(sysm)
+    plan.execute()
+    want = plan.download()
+    for probe, ct, t1 in ((4, 64, 64), (16, 256, 128), (8, 128, 256), (4, 256, 64), (16, 64, 256)):
+        plan.set_tuning(probe, ct, t1)
+        plan.execute()
+        got = plan.download()
+        assert all(np.array_equal(a, b) for a, b in zip(want, got)), (probe, ct, t1)
+    plan.close()
+
+
+def test_autotuned_graph_plan_keeps_results():
+    from libgrpp_amd import lib as G
+    sysm = configs.heavy_cluster()
+    eager = G.Plan(sysm)
+    eager.execute()
+    want = eager.download()
+    eager.close()
+    plan = G.Plan(sysm, G.BATCH_GRAPH)
+    plan.execute()                       # captures the default shapes
+    t = plan.autotune(repeats=1)         # eager timing runs; the graph is recaptured afterwards
+    print("tuned:", t, flush=True)
+    assert t["screen_probe"] in (4, 8, 16) and t["contract_threads"] in (64, 128, 256)
+    assert t["seconds_after"] > 0.0
+    for _ in range(2):
+        plan.execute()
+        assert all(np.array_equal(a, b) for a, b in zip(want, plan.download()))
+    with pytest.raises(RuntimeError, match="tuning"):
+        plan.set_tuning(5, 128, 256)
+    plan.close()
