@@ -1780,7 +1780,8 @@ struct eng_plan {
     int n_delta_centers = 0;
     WorkItem *d_items = nullptr;
     int n_items = 0;
-    PartItem *d_parts = nullptr;     // (item, centre) workgroups of the contraction
+    PartItem *d_parts = nullptr;     // (item, centre) workgroups of the contraction, item order
+    PartItem *d_parts_run = nullptr; // the same parts in launch order: largest blocks first
     int *d_item_part0 = nullptr;     // first part of each item
     double *d_partial = nullptr;     // per-part type-1 blocks (k_type1)
     double *d_partial_ct = nullptr;  // per-part semilocal, spin-orbit and outercore blocks (k_contract)
@@ -2230,6 +2231,17 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     rc |= dput(P, &P->d_delta_centers, delta_centers);
     rc |= dput(P, &P->d_items, witems);
     rc |= dput(P, &P->d_parts, partlist);
+    {
+        // launch order of type-1 and the contraction: the costliest (largest Cartesian blocks)
+        // first, so that they do not form the tail of the launch; results do not depend on it
+        std::vector<PartItem> run(partlist);
+        auto cost = [&](const PartItem &q) {
+            const int la = sh[items[q.item].a].L, lb = sh[items[q.item].b].L;
+            return (long long) ncart(la) * ncart(lb) * (la + lb + 1);
+        };
+        std::stable_sort(run.begin(), run.end(), [&](const PartItem &x, const PartItem &y) { return cost(x) > cost(y); });
+        rc |= dput(P, &P->d_parts_run, run);
+    }
     rc |= dput(P, &P->d_item_part0, item_part0);
     rc |= dalloc(P, &P->d_partial, (size_t) partial_len);
     rc |= dalloc(P, &P->d_partial_ct, (size_t) partial_len);
@@ -2405,7 +2417,7 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
     if (P->n_parts > 0)
         hipLaunchKernelGGL(k_type1, dim3(P->n_parts), dim3(P->tune.type1_threads),
                            (size_t) (P->t1_scratch + P->t1_acc) * sizeof(double),
-                           sd, P->d_parts, P->d_items, P->J, P->T, P->d_partial, P->parts, P->t1_scratch);
+                           sd, P->d_parts_run, P->d_items, P->J, P->T, P->d_partial, P->parts, P->t1_scratch);
     HIPCHK(hipEventRecord(P->evs[2], sd));
     HIPCHK(hipEventRecord(P->ev[1], st));
     if (P->n_tiles > 0) {
@@ -2438,7 +2450,7 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
     HIPCHK(hipEventRecord(P->ev[6], st));
     if (P->n_parts > 0)
         hipLaunchKernelGGL(contract_kernel(P->tune.contract_threads), dim3(P->n_parts), dim3(P->tune.contract_threads),
-                           (size_t) (P->ct_qs + P->ct_ts) * sizeof(double), st, P->d_parts, P->d_items, P->J, P->T,
+                           (size_t) (P->ct_qs + P->ct_ts) * sizeof(double), st, P->d_parts_run, P->d_items, P->J, P->T,
                            P->d_partial_ct, P->parts, P->nm_max, P->ct_qs, P->ct_ts);
     HIPCHK(hipStreamWaitEvent(st, P->evs[2], 0));   // join: the reduction reads the type-1 slices
     if (P->n_items > 0)   // also writes the zero blocks of items without centres
