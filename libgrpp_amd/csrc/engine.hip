@@ -840,7 +840,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
                                                      double *__restrict__ Q, int min_level, double tol,
                                                      unsigned long long *__restrict__ counters,
                                                      const double *__restrict__ scr, const DShell *__restrict__ J_sh,
-                                                     const int *__restrict__ pos, signed char *__restrict__ lev_out)
+                                                     const int *__restrict__ pos, signed char *__restrict__ lev_out,
+                                                     int *__restrict__ tile_lev)
 {
     __shared__ double As[RK][RT + 2];
     __shared__ double Bs[RK][RT + 2];
@@ -952,6 +953,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
         lev_done = lev;
         if (!__syncthreads_or(done != 0xFFFFu)) break;
     }
+    if (tid == 0 && tile_lev) tile_lev[t.pad] = lev_done;   // the tile's last level, for the next launch order
     if (tid == 0 && counters) {
         const int nr = min(RT, jb.M - t.ti * RT), nc = min(RT, jb.M - t.tj * RT);
         const unsigned long long ent = (t.ti == t.tj) ? (unsigned long long) nr * (nr + 1) / 2
@@ -1771,7 +1773,11 @@ struct eng_plan {
     RowDesc *d_rows = nullptr;
     QJob *d_jobs = nullptr;
     TileItem *d_tiles = nullptr;
+    TileItem *d_tiles_radial = nullptr;   // the radial product's launch order (TileItem.pad = tile index)
+    int *d_tile_lev = nullptr;            // last level of every tile in the latest build
+    bool radial_order_fresh = false;      // launch order taken from a build with the current rows
     int n_tiles = 0;
+    std::vector<TileItem> tiles_host;
     ProjItem *d_proj = nullptr;
     int n_proj = 0;
     OcvItem *d_ocvi = nullptr;
@@ -2225,7 +2231,11 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     rc |= dput(P, &P->d_wpots, wpots);
     rc |= dput(P, &P->d_rows, rows);
     rc |= dput(P, &P->d_jobs, jobs);
+    for (size_t i = 0; i < tiles.size(); i++) tiles[i].pad = (int) i;
     rc |= dput(P, &P->d_tiles, tiles);
+    rc |= dput(P, &P->d_tiles_radial, tiles);
+    rc |= dalloc(P, &P->d_tile_lev, tiles.size());
+    P->tiles_host = tiles;
     rc |= dput(P, &P->d_proj, proj);
     rc |= dput(P, &P->d_ocvi, ocv);
     rc |= dput(P, &P->d_delta_centers, delta_centers);
@@ -2429,13 +2439,13 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
     HIPCHK(hipEventRecord(P->ev[2], st));
     if (P->n_tiles > 0) {
         if (P->radial_mfma && P->d_lev)
-            hipLaunchKernelGGL(k_radial_mfma<true>, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs,
+            hipLaunchKernelGGL(k_radial_mfma<true>, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles_radial, P->d_jobs,
                                P->d_rows, P->d_F, P->T.rpow, P->d_wU, P->d_Q, P->min_level, P->tol, P->d_counters,
-                               P->d_scr, P->J.sh, P->d_rowpos, P->d_lev);
+                               P->d_scr, P->J.sh, P->d_rowpos, P->d_lev, P->d_tile_lev);
         else if (P->radial_mfma)
-            hipLaunchKernelGGL(k_radial_mfma<false>, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs,
+            hipLaunchKernelGGL(k_radial_mfma<false>, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles_radial, P->d_jobs,
                                P->d_rows, P->d_F, P->T.rpow, P->d_wU, P->d_Q, P->min_level, P->tol, P->d_counters,
-                               P->d_scr, P->J.sh, P->d_rowpos, P->d_lev);
+                               P->d_scr, P->J.sh, P->d_rowpos, P->d_lev, P->d_tile_lev);
         else
             hipLaunchKernelGGL(k_radial, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->d_F,
                                P->T.rpow, P->d_wU, P->d_Q, P->min_level, P->tol, P->d_counters, P->d_scr, P->J.sh,
@@ -2504,6 +2514,19 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
         HIPCHK(hipEventSynchronize(P->ev[7]));
         const int rt = eager_phase_times(P);
         if (rt) return rt;
+    }
+    if (!P->radial_order_fresh && P->n_tiles > 0 && P->radial_mfma) {
+        // the tiles that ran to the highest levels go first in the next builds (no result depends
+        // on the order); refreshed after every change of the row order
+        std::vector<int> lev(P->n_tiles);
+        HIPCHK(hipMemcpy(lev.data(), P->d_tile_lev, lev.size() * sizeof(int), hipMemcpyDeviceToHost));
+        std::vector<int> idx(P->n_tiles);
+        for (int i = 0; i < P->n_tiles; i++) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return lev[a] > lev[b]; });
+        std::vector<TileItem> order(P->n_tiles);
+        for (int i = 0; i < P->n_tiles; i++) order[i] = P->tiles_host[(size_t) idx[i]];
+        HIPCHK(hipMemcpy(P->d_tiles_radial, order.data(), order.size() * sizeof(TileItem), hipMemcpyHostToDevice));
+        P->radial_order_fresh = true;
     }
     unsigned long long cnt[8];
     HIPCHK(hipMemcpy(cnt, P->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
@@ -2684,6 +2707,7 @@ extern "C" int eng_plan_set_predictions(eng_plan_t *P, int n, const float *pred)
     }
     HIPCHK(hipMemcpy(P->d_rows, phys.data(), phys.size() * sizeof(RowDesc), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(P->d_rowpos, P->rowpos.data(), P->rowpos.size() * sizeof(int), hipMemcpyHostToDevice));
+    P->radial_order_fresh = false;
     return 0;
 }
 
