@@ -518,7 +518,7 @@ int libgrpp_batch_plan_autotune(libgrpp_batch_plan_t *p, int repeats, libgrpp_ba
     double before = 0.0;
     int rc = time_builds(p, repeats, 0, &before);
     if (rc) return rc;
-    /* one parameter at a time, each judged by the phase it acts on */
+    /* one parameter at a time, each timed by the phase it acts on */
     static const int probes[3] = {4, 8, 16}, threads[3] = {64, 128, 256};
     for (int par = 0; par < 3; par++) {
         int *field = par == 0 ? &t.screen_probe : par == 1 ? &t.contract_threads : &t.type1_threads;
