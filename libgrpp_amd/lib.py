@@ -113,13 +113,14 @@ class _Objects:
         self.grpps = (PGRPP * max(1, nc))()
         self.xyz = np.zeros(3 * max(1, nc))
         for i, (o, g) in enumerate(sysm.centers):
-            self.grpps[i] = self.f.grpp(g)
+            self.grpps[i] = self.f.grpp(g) if g is not None else None   # None: a centre without GRPP
             self.xyz[3 * i:3 * i + 3] = o
 
     def close(self):
         self.f.free_shells(self.shells, len(self.sysm.shells))
-        for i in range(len(self.sysm.centers)):
-            self.f.free_grpp(self.grpps[i])
+        for i, (_, g) in enumerate(self.sysm.centers):
+            if g is not None:
+                self.f.free_grpp(self.grpps[i])
 
 
 def batch_matrices(sysm: System, flags: int = 0):
