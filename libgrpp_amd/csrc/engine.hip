@@ -1431,7 +1431,7 @@ __device__ __forceinline__ double int_gauss(int n, double a)
     return fk / (2.0 * pow(a, k + 1));
 }
 
-constexpr int TS_MAX = 6144;   // upper bound of the LDS doubles for the (m, i, k2) intermediate
+constexpr int TS_MAX = 2048;   // upper bound of the LDS doubles for the (m, i, k2) intermediate
 
 // One workgroup per (shell pair, centre) (PartItem): the centre's contribution to the four blocks
 // goes to its own slice of the contraction's partial buffer; k_contract_reduce adds, in centre
