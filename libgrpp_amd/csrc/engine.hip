@@ -1763,7 +1763,8 @@ struct eng_plan {
     long long out_len = 0, op_stride = 0;
     DevTables T{};
     DevJob J{};
-    std::vector<void *> bufs;
+    int device = 0;
+    std::vector<std::pair<void *, size_t>> bufs;   // device buffers and their pooled sizes
     double *d_out = nullptr, *d_F = nullptr, *d_wU = nullptr, *d_Q = nullptr, *d_P = nullptr, *d_OCV = nullptr,
            *d_delta = nullptr, *d_scr = nullptr, *d_Qc = nullptr;
     TabItem *d_tab = nullptr;
@@ -1827,11 +1828,53 @@ struct eng_plan {
     long long qlen = 0;
 };
 
+// Device buffers of small plans are recycled: a destroyed plan returns buffers of up to
+// POOL_MAX_BUFFER bytes to a per-device free list (by size, rounded up to 256 bytes, at most
+// POOL_MAX_TOTAL bytes held), so the per-call entry points, which build and destroy a one-task plan
+// per call, do not pay hipMalloc / hipFree each time.  Contents are not cleared (hipMalloc does not
+// clear either); every buffer a build reads is written before it is read.
+constexpr size_t POOL_MAX_BUFFER = (size_t) 16 << 20, POOL_MAX_TOTAL = (size_t) 512 << 20;
+static std::mutex g_pool_mutex;
+static std::multimap<std::pair<int, size_t>, void *> g_pool;
+static size_t g_pool_bytes = 0;
+
+static int pool_alloc(int dev, size_t bytes, void **p)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mutex);
+        auto it = g_pool.find({dev, bytes});
+        if (it != g_pool.end()) {
+            *p = it->second;
+            g_pool.erase(it);
+            g_pool_bytes -= bytes;
+            return 0;
+        }
+    }
+    HIPCHK(hipMalloc(p, bytes));
+    return 0;
+}
+
+static void pool_free(int dev, void *p, size_t bytes)
+{
+    if (bytes <= POOL_MAX_BUFFER) {
+        std::lock_guard<std::mutex> lk(g_pool_mutex);
+        if (g_pool_bytes + bytes <= POOL_MAX_TOTAL) {
+            g_pool.insert({{dev, bytes}, p});
+            g_pool_bytes += bytes;
+            return;
+        }
+    }
+    (void) hipFree(p);
+}
+
 template <class T>
 static int dalloc(eng_plan *P, T **p, size_t n)
 {
-    HIPCHK(hipMalloc((void **) p, std::max<size_t>(1, n) * sizeof(T)));
-    P->bufs.push_back(*p);
+    const size_t bytes = (std::max<size_t>(1, n) * sizeof(T) + 255) & ~(size_t) 255;
+    void *q = nullptr;
+    if (pool_alloc(P->device, bytes, &q)) return 1;
+    *p = (T *) q;
+    P->bufs.push_back({q, bytes});
     return 0;
 }
 
@@ -1857,6 +1900,7 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     *plan_out = nullptr;
     if (!eng_device_ok()) return 1;
     eng_plan *P = new eng_plan();
+    HIPCHK(hipGetDevice(&P->device));
     P->nshell = nshell;
     P->ncenter = ncenter;
     P->parts = parts;
@@ -2565,7 +2609,7 @@ extern "C" double *eng_plan_device_output(eng_plan_t *P) { return P->d_out; }
 extern "C" void eng_plan_destroy(eng_plan_t *P)
 {
     if (!P) return;
-    for (void *b : P->bufs) (void) hipFree(b);
+    for (auto &b : P->bufs) pool_free(P->device, b.first, b.second);
     for (int i = 0; i < 8; i++)
         if (P->ev[i]) (void) hipEventDestroy(P->ev[i]);
     for (int i = 0; i < 3; i++)
