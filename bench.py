@@ -109,7 +109,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="time the plain method only")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuning)")
     ap.add_argument("--tuning", default=None,
-                    help="fixed launch shapes screen_probe,contract_threads,type1_threads instead of autotuning")
+                    help="fixed launch shapes screen_probe,contract_threads,type1_threads[,stream_layout] instead of autotuning")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device-to-device copy (3 times): the known-size kernel "
@@ -155,9 +155,11 @@ def main():
     def tune(plan):
         """the plan's launch shapes picked by timing its own builds (untimed; results unchanged)"""
         if args.tuning:
-            probe, ct, t1 = (int(x) for x in args.tuning.split(","))
-            plan.set_tuning(probe, ct, t1)
-            return {"screen_probe": probe, "contract_threads": ct, "type1_threads": t1, "fixed": True}
+            v = [int(x) for x in args.tuning.split(",")]
+            probe, ct, t1, lay = (v + [0])[:4]
+            plan.set_tuning(probe, ct, t1, lay)
+            return {"screen_probe": probe, "contract_threads": ct, "type1_threads": t1, "stream_layout": lay,
+                    "fixed": True}
         if args.no_tune:
             return None
         plan.execute()

@@ -490,7 +490,7 @@ int libgrpp_batch_plan_dim(const libgrpp_batch_plan_t *p) { return p->dim; }
 
 int libgrpp_batch_plan_set_tuning(libgrpp_batch_plan_t *p, const libgrpp_batch_tuning_t *t)
 {
-    eng_tuning_t e = {t->screen_probe, t->contract_threads, t->type1_threads};
+    eng_tuning_t e = {t->screen_probe, t->contract_threads, t->type1_threads, t->stream_layout};
     return eng_plan_set_tuning(p->eng, &e);
 }
 
@@ -519,13 +519,15 @@ int libgrpp_batch_plan_autotune(libgrpp_batch_plan_t *p, int repeats, libgrpp_ba
     int rc = time_builds(p, repeats, 0, &before);
     if (rc) return rc;
     /* one parameter at a time, each timed by the phase it acts on */
-    static const int probes[3] = {4, 8, 16}, threads[3] = {64, 128, 256};
-    for (int par = 0; par < 3; par++) {
-        int *field = par == 0 ? &t.screen_probe : par == 1 ? &t.contract_threads : &t.type1_threads;
-        const int *cand = par == 0 ? probes : threads;
+    static const int probes[3] = {4, 8, 16}, threads[3] = {64, 128, 256}, layouts[2] = {0, 1};
+    for (int par = 0; par < 4; par++) {
+        int *field = par == 0 ? &t.screen_probe : par == 1 ? &t.contract_threads : par == 2 ? &t.type1_threads
+                                                                                           : &t.layout;
+        const int *cand = par == 0 ? probes : par == 3 ? layouts : threads;
+        const int ncand = par == 3 ? 2 : 3;
         int best_v = *field;
         double best_t = 1e300;
-        for (int c = 0; c < 3; c++) {
+        for (int c = 0; c < ncand; c++) {
             *field = cand[c];
             if ((rc = eng_plan_set_tuning(p->eng, &t))) return rc;
             double v;
@@ -544,6 +546,7 @@ int libgrpp_batch_plan_autotune(libgrpp_batch_plan_t *p, int repeats, libgrpp_ba
         chosen->screen_probe = t.screen_probe;
         chosen->contract_threads = t.contract_threads;
         chosen->type1_threads = t.type1_threads;
+        chosen->stream_layout = t.layout;
         chosen->seconds_before = before;
         chosen->seconds_after = after;
     }

@@ -49,6 +49,8 @@ typedef struct {
     int screen_probe;       /* triples per entry in the screening probe: 4, 8 or 16 */
     int contract_threads;   /* k_contract workgroup: 64, 128 or 256 threads */
     int type1_threads;      /* k_type1 workgroup: 64, 128 or 256 threads */
+    int layout;             /* 0: tables, projectors, type-1 on the side stream; 1: tables and projectors
+                               ahead of the screening, type-1 alone on the side stream */
 } eng_tuning_t;
 
 /* flags */

@@ -1794,7 +1794,7 @@ struct eng_plan {
     double *d_partial_ct = nullptr;  // per-part semilocal, spin-orbit and outercore blocks (k_contract)
     int n_parts = 0;
     unsigned long long *d_counters = nullptr;
-    eng_tuning_t tune{8, 128, 256};   // launch shapes; none of them changes a result
+    eng_tuning_t tune{8, 128, 256, 0};   // launch shapes and stream layout; none of them changes a result
     hipEvent_t ev[8] = {};   // execution phases on the caller's stream: see eng_plan_execute   // execution phases on the caller's stream: see eng_plan_execute
     hipStream_t side = nullptr;   // second stream (default priority) for the radial-independent kernels
     hipStream_t cap = nullptr;    // capture stream of a graph plan (default priority)
@@ -1802,6 +1802,7 @@ struct eng_plan {
     hipGraphExec_t gexec = nullptr;
     hipEvent_t gev[2] = {};       // around a graph replay
     hipEvent_t evs[3] = {};       // phases on the side stream
+    hipEvent_t evt1 = nullptr;    // start of type-1
     // statistics
     long long n_tasks = 0, n_integrals = 0, n_entries = 0;
     double flops_fixed = 0.0;    // contraction + type-1 + projector work (grid independent)
@@ -2409,6 +2410,7 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     J.op_stride = op_stride;
     for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&P->ev[i]));
     for (int i = 0; i < 3; i++) HIPCHK(hipEventCreate(&P->evs[i]));
+    HIPCHK(hipEventCreate(&P->evt1));
     HIPCHK(hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking));
     if (flags & ENG_F_GRAPH) {
         HIPCHK(hipStreamCreateWithFlags(&P->cap, hipStreamNonBlocking));
@@ -2444,13 +2446,25 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
     // radial product, canonical splits) and the contraction; the side stream runs what does not
     // depend on it (grid tables, projectors, outercore vectors, type-1), forked after the counters
     // are cleared.  The radial product waits for the grid tables (ev 5), the contraction for the
-    // projectors (s1), the final reduction for the type-1 slices (s2).
+    // projectors (s1), the final reduction for the type-1 slices (s2).  Layout 1 (an autotuner
+    // choice) runs the tables, projectors and outercore vectors on the caller's stream ahead of the
+    // screening instead, and type-1 alone on the side stream from the start.
     // Events: caller 0 start = 1 | screening | 2 | radial | 3 | canonical | 4 ... 6 | contraction,
-    // reduction | 7; side s0 | tables | 5 | projectors, outercore | s1 | type-1 | s2.
+    // reduction | 7; s0 | tables | 5 | projectors, outercore | s1 on the tables' stream;
+    // side t1 | type-1 | s2.
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, P->d_counters, 8);
     HIPCHK(hipEventRecord(P->ev[0], st));
     HIPCHK(hipStreamWaitEvent(P->side, P->ev[0], 0));
-    hipStream_t sd = P->side;
+    const bool front = P->tune.layout == 1;
+    hipStream_t sd = front ? st : P->side;
+    if (front) {   // type-1 needs no table: the side stream starts with it
+        HIPCHK(hipEventRecord(P->evt1, P->side));
+        if (P->n_parts > 0)
+            hipLaunchKernelGGL(k_type1, dim3(P->n_parts), dim3(P->tune.type1_threads),
+                               (size_t) (P->t1_scratch + P->t1_acc) * sizeof(double), P->side, P->d_parts_run,
+                               P->d_items, P->J, P->T, P->d_partial, P->parts, P->t1_scratch);
+        HIPCHK(hipEventRecord(P->evs[2], P->side));
+    }
     HIPCHK(hipEventRecord(P->evs[0], sd));
     // grid tables first: the radial product waits for them (event ev[5]); the screening does not
     if (P->n_tab > 0) {
@@ -2468,11 +2482,14 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
                            P->n_delta_centers, P->J, P->d_delta);
     if (P->n_ocv > 0) hipLaunchKernelGGL(k_ocv, dim3(P->n_ocv), dim3(256), 0, sd, P->d_ocvi, P->J, P->T, P->d_OCV);
     HIPCHK(hipEventRecord(P->evs[1], sd));
-    if (P->n_parts > 0)
-        hipLaunchKernelGGL(k_type1, dim3(P->n_parts), dim3(P->tune.type1_threads),
-                           (size_t) (P->t1_scratch + P->t1_acc) * sizeof(double),
-                           sd, P->d_parts_run, P->d_items, P->J, P->T, P->d_partial, P->parts, P->t1_scratch);
-    HIPCHK(hipEventRecord(P->evs[2], sd));
+    if (!front) {
+        HIPCHK(hipEventRecord(P->evt1, sd));
+        if (P->n_parts > 0)
+            hipLaunchKernelGGL(k_type1, dim3(P->n_parts), dim3(P->tune.type1_threads),
+                               (size_t) (P->t1_scratch + P->t1_acc) * sizeof(double),
+                               sd, P->d_parts_run, P->d_items, P->J, P->T, P->d_partial, P->parts, P->t1_scratch);
+        HIPCHK(hipEventRecord(P->evs[2], sd));
+    }
     HIPCHK(hipEventRecord(P->ev[1], st));
     if (P->n_tiles > 0) {
         auto screen = P->tune.screen_probe == 4 ? k_screen<4> : P->tune.screen_probe == 16 ? k_screen<16> : k_screen<8>;
@@ -2584,7 +2601,7 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
 
 static int eager_phase_times(eng_plan_t *P)
 {
-    const hipEvent_t *pb[7] = {&P->evs[0], &P->ev[1], &P->ev[2], &P->ev[3], &P->ev[5], &P->evs[1], &P->ev[6]};
+    const hipEvent_t *pb[7] = {&P->evs[0], &P->ev[1], &P->ev[2], &P->ev[3], &P->ev[5], &P->evt1, &P->ev[6]};
     const hipEvent_t *pe[7] = {&P->ev[5], &P->ev[2], &P->ev[3], &P->ev[4], &P->evs[1], &P->evs[2], &P->ev[7]};
     for (int i = 0; i < 7; i++) {
         float ms = 0;
@@ -2614,6 +2631,7 @@ extern "C" void eng_plan_destroy(eng_plan_t *P)
         if (P->ev[i]) (void) hipEventDestroy(P->ev[i]);
     for (int i = 0; i < 3; i++)
         if (P->evs[i]) (void) hipEventDestroy(P->evs[i]);
+    if (P->evt1) (void) hipEventDestroy(P->evt1);
     if (P->side) (void) hipStreamDestroy(P->side);
     if (P->gexec) (void) hipGraphExecDestroy(P->gexec);
     if (P->graph) (void) hipGraphDestroy(P->graph);
@@ -2669,8 +2687,8 @@ extern "C" int eng_plan_set_tuning(eng_plan_t *P, const eng_tuning_t *t)
 {
     auto in3 = [](int v, int a, int b, int c) { return v == a || v == b || v == c; };
     if (!in3(t->screen_probe, 4, 8, 16) || !in3(t->contract_threads, 64, 128, 256) ||
-        !in3(t->type1_threads, 64, 128, 256)) {
-        eng_set_error("tuning: screen_probe must be 4, 8 or 16 and the thread counts 64, 128 or 256");
+        !in3(t->type1_threads, 64, 128, 256) || (t->layout != 0 && t->layout != 1)) {
+        eng_set_error("tuning: screen_probe must be 4, 8 or 16, the thread counts 64, 128 or 256, the layout 0 or 1");
         return 2;
     }
     P->tune = *t;

@@ -1,6 +1,6 @@
 """The autotuner changes launch shapes only: every shape it may pick gives bitwise the matrices of
 the default shapes (screening probe width 4/8/16, contraction and type-1 workgroups of 64/128/256
-threads), and a tuned plan, eager or graph-replayed, still does."""
+threads, either stream layout), and a tuned plan, eager or graph-replayed, still does."""
 import numpy as np
 import pytest
 
@@ -16,11 +16,11 @@ def test_every_launch_shape_gives_the_same_matrices(workload):
     plan = G.Plan(sysm)
     plan.execute()
     want = plan.download()
-    for probe, ct, t1 in ((4, 64, 64), (16, 256, 128), (8, 128, 256), (4, 256, 64), (16, 64, 256)):
-        plan.set_tuning(probe, ct, t1)
+    for probe, ct, t1, lay in ((4, 64, 64, 1), (16, 256, 128, 0), (8, 128, 256, 1), (4, 256, 64, 0), (16, 64, 256, 1)):
+        plan.set_tuning(probe, ct, t1, lay)
         plan.execute()
         got = plan.download()
-        assert all(np.array_equal(a, b) for a, b in zip(want, got)), (probe, ct, t1)
+        assert all(np.array_equal(a, b) for a, b in zip(want, got)), (probe, ct, t1, lay)
     plan.close()
 
 
@@ -41,5 +41,5 @@ def test_autotuned_graph_plan_keeps_results():
         plan.execute()
         assert all(np.array_equal(a, b) for a, b in zip(want, plan.download()))
     with pytest.raises(RuntimeError, match="tuning"):
-        plan.set_tuning(5, 128, 256)
+        plan.set_tuning(5, 128, 256, 0)
     plan.close()
