@@ -111,6 +111,8 @@ def main():
     ap.add_argument("--tuning", default=None,
                     help="fixed launch shapes screen_probe,contract_threads,type1_threads[,stream_layout] instead of autotuning")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--graph", action="store_true",
+                    help="also time the plain build captured once as a hipGraph and replayed (graph_leg)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device-to-device copy (3 times): the known-size kernel "
                          "against which the FETCH_SIZE / WRITE_SIZE counters are scaled")
@@ -172,6 +174,28 @@ def main():
     wall_plain, st_plain, mean_plain = timed(plan)
     plain_out = plan.download()
     plan.close()
+    graph_info = None
+    if args.graph:
+        # the same plain build captured into one hipGraph on its first execute, then replayed
+        gplan = LG.make_plan(sysm, G.BATCH_GRAPH)
+        if tuning_plain:
+            gplan.set_tuning(tuning_plain["screen_probe"], tuning_plain["contract_threads"],
+                             tuning_plain["type1_threads"], tuning_plain["stream_layout"])
+        for _ in range(max(1, args.warmup)):
+            gplan.execute()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            gplan.execute()
+        torch.cuda.synchronize()
+        gwall = time.perf_counter() - t0
+        gout = gplan.download()
+        gplan.close()
+        flat = lambda ms: [m for x in ms for m in (x if isinstance(x, list) else [x])]
+        graph_info = {"graph_ms_per_step": 1e3 * gwall / args.steps,
+                      "eager_ms_per_step": 1e3 * wall_plain / args.steps,
+                      "bitwise_equal_to_eager": all(np.array_equal(x, y)
+                                                    for x, y in zip(flat(gout), flat(plain_out)))}
     # learned grid levels: the predictor is trained (untimed) on heavy_cluster seeds 1-4, then its
     # BF16 inference orders the radial tables of the seed-0 workload
     learned_info = None
@@ -247,6 +271,8 @@ def main():
     if learned_info:
         line["learned_grid"] = learned_info
     line["autotune"] = learned_info["autotune_learned"] if learned_info else tuning_plain
+    if graph_info:
+        line["graph_leg"] = graph_info
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(sysm, args.cpu_budget)
     print(json.dumps(line))
