@@ -8,7 +8,8 @@ namespace grpp {
 
 struct DevTables {
     const double *r, *w, *rpow, *bes, *rfun, *boys;
-    const double *besh;       // Taylor coefficients over k! (Horner form), steering-only evaluations
+    const double *besp;       // Taylor coefficients over k! (Horner form) of K_l and K_{l-1} per node and l,
+                              // 80 bytes each: steering-only evaluations
     const double *rsh;
     const int *rsh_off, *comb, *comb_off;
     const double *lmat;

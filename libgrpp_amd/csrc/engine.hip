@@ -349,8 +349,10 @@ __device__ __forceinline__ double bessel_k_steer(const DevTables &T, int n, doub
     if (x > 1e-7 && x <= 16.0) {
         const int i0 = (int) (rint(x * 100) + 1e-5);
         const double dx = x - i0 * 0.01;
-        const double *d = T.besh + ((size_t) i0 * BES_NL + n) * 5;
-        return d[0] + dx * (d[1] + dx * (d[2] + dx * (d[3] + dx * d[4])));
+        const double2 *d = (const double2 *) (T.besp + ((size_t) i0 * BES_NL + n) * 10);
+        const double2 c01 = d[0], c23 = d[1];
+        const double c4 = d[2].x;
+        return c01.x + dx * (c01.y + dx * (c23.x + dx * (c23.y + dx * c4)));
     }
     if (x > 16.0) {
         const double inv = -rcp_refined(x);
@@ -370,9 +372,11 @@ __device__ __forceinline__ double bessel_ratio(const DevTables &T, int l, double
     if (x > 1e-7 && x <= 16.0) {
         const int i0 = (int) (rint(x * 100) + 1e-5);
         const double dx = x - i0 * 0.01;
-        const double *da = T.besh + ((size_t) i0 * BES_NL + a) * 5, *db = T.besh + ((size_t) i0 * BES_NL + b) * 5;
-        const double ka = da[0] + dx * (da[1] + dx * (da[2] + dx * (da[3] + dx * da[4])));
-        const double kb = db[0] + dx * (db[1] + dx * (db[2] + dx * (db[3] + dx * db[4])));
+        // entry l holds K_b's coefficients, then K_a's: five 16-byte loads per lane
+        const double2 *d = (const double2 *) (T.besp + ((size_t) i0 * BES_NL + l) * 10);
+        const double2 b01 = d[0], b23 = d[1], b4a0 = d[2], a12 = d[3], a34 = d[4];
+        const double ka = b4a0.y + dx * (a12.x + dx * (a12.y + dx * (a34.x + dx * a34.y)));
+        const double kb = b01.x + dx * (b01.y + dx * (b23.x + dx * (b23.y + dx * b4a0.x)));
         return ka * rcp_refined(kb);
     }
     if (x > 16.0) {
@@ -1697,8 +1701,8 @@ static int device_tables(int Lmax, int nmax, DevTables *out)
         D.t.rpow = p;
         if (upload(H.bes, &p, D.bufs)) return 1;
         D.t.bes = p;
-        if (upload(H.besh, &p, D.bufs)) return 1;
-        D.t.besh = p;
+        if (upload(H.besp, &p, D.bufs)) return 1;
+        D.t.besp = p;
         if (upload(H.rfun, &p, D.bufs)) return 1;
         D.t.rfun = p;
         if (upload(H.boys, &p, D.bufs)) return 1;

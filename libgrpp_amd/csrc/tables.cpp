@@ -157,10 +157,20 @@ static void build(HostTables &T)
     for (int i = 0; i < BES_NPT; i++)
         for (int l = 0; l < BES_NL; l++)
             for (int k = 0; k < 5; k++) T.bes[((size_t) i * BES_NL + l) * 5 + k] = F(i, l, k);
-    T.besh.assign(T.bes.size(), 0.0);
+    // both orders of a Bessel ratio in one 80-byte entry (16-byte aligned): the steering
+    // evaluations gather them with five 16-byte loads per lane
+    T.besp.assign((size_t) BES_NPT * BES_NL * 10, 0.0);
     {
         const double inv_fact[5] = {1.0, 1.0, 0.5, 1.0 / 6.0, 1.0 / 24.0};
-        for (size_t e = 0; e < T.bes.size(); e++) T.besh[e] = T.bes[e] * inv_fact[e % 5];
+        for (int i = 0; i < BES_NPT; i++)
+            for (int l = 0; l < BES_NL; l++) {
+                const int lo = l > 0 ? l - 1 : 1;
+                double *e = &T.besp[((size_t) i * BES_NL + l) * 10];
+                for (int k = 0; k < 5; k++) {
+                    e[k] = F(i, l, k) * inv_fact[k];
+                    e[5 + k] = F(i, lo, k) * inv_fact[k];
+                }
+            }
     }
     T.rfun.assign(BES_NL * BES_NL, 0.0);
     for (int n = 0; n < BES_NL; n++)

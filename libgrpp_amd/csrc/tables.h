@@ -21,7 +21,8 @@ struct HostTables {
     std::vector<double> r, w;                 // [NPTS]
     std::vector<double> rpow;                 // [(MAXPOW+1) * NPTS]
     std::vector<double> bes;                  // [BES_NPT][BES_NL][5]
-    std::vector<double> besh;                 // [BES_NPT][BES_NL][5]: Taylor coefficient k over k! (Horner form)
+    std::vector<double> besp;                 // [BES_NPT][BES_NL][10]: Taylor coefficient k over k! (Horner form)
+                                              // of K_l (0..4), then of K_{l-1} (K_1 for l = 0) (5..9)
     std::vector<double> rfun;                 // [BES_NL][BES_NL]
     std::vector<double> boys;                 // [BOYS_NPT][BOYS_NN]
     // real spherical harmonics y_{lm}^{ijk}: per l, ncomb = (l+1)(l+2)/2 combos (i ascending, j, k)
