@@ -144,8 +144,9 @@ struct DevJob {
     // lookup tables, slot = (centre, L) pair
     const int *slot_rowoff;        // [slot * nshell + A]
     const int *rowpos;             // [slot row base + logical row] -> physical row of the radial table
-    const int *pperm;              // [poff + i] -> primitive of the shell with the i-th smallest exponent
-    const int *tperm;              // [toff + t] -> potential term with the t-th smallest exponent
+    const double2 *pprobe;         // [poff + i] {alpha, cnorm} of the shell's primitive with the i-th smallest exponent
+    const double2 *tprobe;         // [toff + t] {zeta, coef} of the potential's term with the t-th smallest exponent
+    const int *tpow_probe;         // [toff + t] its power n
     const long long *slot_poff;    // [slot * nshell + A]
     const long long *slot_ploff;   // [slot * nshell + A] (-1 if the slot has no SO channel)
     const long long *slot_pb2;     // bra projector, type-2 thresholds
@@ -450,12 +451,13 @@ __device__ __forceinline__ bool screen_probe_triple(const DevJob &J, const DevTa
 {
     const int nt = P.nterm, njt = B.nprim * nt;
     const int iq = q / njt, r = q - iq * njt, jq = r / nt, tq = r - jq * nt;
-    const int i = J.pperm[A.poff + iq], j = J.pperm[B.poff + jq], t = J.tperm[P.toff + tq];
-    const double aA = J.alpha[i], aB = J.alpha[j], k1 = 2 * aA * CA, k2 = 2 * aB * CB;
-    active = J.cnorm[i] != 0.0 && J.cnorm[j] != 0.0 && !((l1 >= 1 && fabs(k1) <= 1e-14) || (l2 >= 1 && fabs(k2) <= 1e-14));
+    const double2 pa = J.pprobe[A.poff + iq], pb = J.pprobe[B.poff + jq];
+    const double aA = pa.x, aB = pb.x, k1 = 2 * aA * CA, k2 = 2 * aB * CB;
+    active = pa.y != 0.0 && pb.y != 0.0 && !((l1 >= 1 && fabs(k1) <= 1e-14) || (l2 >= 1 && fabs(k2) <= 1e-14));
     if (!active || (l1 == 0 && l2 == 0 && fabs(k1) <= 1e-14 && fabs(k2) <= 1e-14)) return false;
     double r0;
-    return !screen_fixed_point(T, N + J.tpow[t], l1, l2, aA, aB, J.tzeta[t], CA, CB, k1, k2, r0, rsteps);
+    return !screen_fixed_point(T, N + J.tpow_probe[P.toff + tq], l1, l2, aA, aB, J.tprobe[P.toff + tq].x, CA, CB,
+                               k1, k2, r0, rsteps);
 }
 
 // The envelope of one radial entry (screening.c:193-248), evaluated by a whole wave: lane q takes
@@ -482,19 +484,20 @@ __device__ double screen_entry_wave(const DevJob &J, const DevTables &T, const D
         if (q < ntot) {
             // triples in the order of the screening probe (smallest exponents first)
             const int iq = q / njt, r = q - iq * njt, jq = r / nt, tq = r - jq * nt;
-            const int i = J.pperm[A.poff + iq], j = J.pperm[B.poff + jq], t = J.tperm[P.toff + tq] - P.toff;
-            ci = J.cnorm[i];
-            cj = J.cnorm[j];
-            aA = J.alpha[i];
-            aB = J.alpha[j];
+            const double2 pa = J.pprobe[A.poff + iq], pb = J.pprobe[B.poff + jq];
+            aA = pa.x;
+            ci = pa.y;
+            aB = pb.x;
+            cj = pb.y;
             k1 = 2 * aA * CA;
             k2 = 2 * aB * CB;
             // primitives dropped by libgrpp_shell_shrink; K_l(0) = 0 for l >= 1
             active = ci != 0.0 && cj != 0.0 && !((l1 >= 1 && fabs(k1) <= 1e-14) || (l2 >= 1 && fabs(k2) <= 1e-14));
             if (active) {
-                eta = J.tzeta[P.toff + t];
-                coef = J.tcoef[P.toff + t];
-                n = N + J.tpow[P.toff + t];
+                const double2 tz = J.tprobe[P.toff + tq];
+                eta = tz.x;
+                coef = tz.y;
+                n = N + J.tpow_probe[P.toff + tq];
                 analytic = (l1 == 0 && l2 == 0 && fabs(k1) <= 1e-14 && fabs(k2) <= 1e-14);
                 if (!analytic) fail = !screen_fixed_point(T, n, l1, l2, aA, aB, eta, CA, CB, k1, k2, r0, rsteps);
             }
@@ -2237,7 +2240,8 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     double *d_alpha, *d_craw, *d_cnorm, *d_tcoef, *d_tzeta;
     DCenter *d_ctr;
     DPot *d_pot;
-    int *d_tpow, *d_cidx, *d_slot_rowoff, *d_pperm, *d_tperm;
+    int *d_tpow, *d_cidx, *d_slot_rowoff, *d_tpow_probe;
+    double2 *d_pprobe, *d_tprobe;
     OcEntry *d_oce;
     long long *d_slot_poff, *d_slot_ploff, *d_ocv_tab, *d_slot_pb2, *d_slot_pb3;
     int rc = 0;
@@ -2258,8 +2262,18 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
             std::stable_sort(tperm.begin() + d.toff, tperm.begin() + d.toff + d.nterm,
                              [&](int a, int b) { return tzeta[a] < tzeta[b]; });
         }
-        rc |= dput(P, &d_pperm, pperm);
-        rc |= dput(P, &d_tperm, tperm);
+        // the screening reads the data of its probe order directly: {alpha, cnorm} per primitive,
+        // {zeta, coef} and n per term, one 16-byte load per operand and no index indirection
+        std::vector<double2> pprobe(alpha.size()), tprobe(tzeta.size());
+        std::vector<int> tpow_probe(tzeta.size());
+        for (size_t k = 0; k < pperm.size(); k++) pprobe[k] = make_double2(alpha[pperm[k]], cnorm[pperm[k]]);
+        for (size_t k = 0; k < tperm.size(); k++) {
+            tprobe[k] = make_double2(tzeta[tperm[k]], tcoef[tperm[k]]);
+            tpow_probe[k] = tpow[tperm[k]];
+        }
+        rc |= dput(P, &d_pprobe, pprobe);
+        rc |= dput(P, &d_tprobe, tprobe);
+        rc |= dput(P, &d_tpow_probe, tpow_probe);
     }
     rc |= dput(P, &d_craw, craw);
     rc |= dput(P, &d_cnorm, cnorm);
@@ -2394,8 +2408,9 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     J.pot = d_pot;
     J.tpow = d_tpow;
     J.rowpos = P->d_rowpos;
-    J.pperm = d_pperm;
-    J.tperm = d_tperm;
+    J.pprobe = d_pprobe;
+    J.tprobe = d_tprobe;
+    J.tpow_probe = d_tpow_probe;
     J.tcoef = d_tcoef;
     J.tzeta = d_tzeta;
     J.cidx = d_cidx;
