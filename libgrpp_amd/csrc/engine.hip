@@ -874,7 +874,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     const double *wp = wU + (size_t) jb.pot * NPTS;
 
     v4d acc[2][2];
-    double S[2][2][4], res[2][2][4], est[2][2][4];
+    // S: the running sum of an entry, frozen once the entry is settled (converged or screened out,
+    // then its estimate), so it is also the entry's result
+    double S[2][2][4], est[2][2][4];
     unsigned done = 0;
     int stop[2][2][4];   // stopping level per entry: -1 screened out, 9 not converged by the last level
 #pragma unroll
@@ -884,7 +886,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 S[a][b][r] = 0.0;
-                res[a][b][r] = 0.0;
                 if (REC) stop[a][b][r] = 9;
                 est[a][b][r] = __longlong_as_double(0x7ff8000000000000LL);
                 const int row = t.ti * RT + wr + a * 16 + lk + 4 * r, col = t.tj * RT + wc + b * 16 + li;
@@ -894,12 +895,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
                     est[a][b][r] = scr[jb.qoff + canon_index(R, J_sh, pos, jb, row, col)];
                     if (!isnan(est[a][b][r]) && fabs(est[a][b][r]) < tol) {
                         done |= 1u << (a * 8 + b * 4 + r);
-                        res[a][b][r] = est[a][b][r];
+                        S[a][b][r] = est[a][b][r];
                         if (REC) stop[a][b][r] = -1;
                     }
                 }
             }
 
+    // The K steps of the levels are contiguous in the point order (level l covers points
+    // [16 << l, 32 << l)), so the global operands of the next step are loaded into registers while
+    // the matrix cores work on the current one.
+    constexpr int NW = ((LMAX_SHELL + 1) * RK + 255) / 256;
+    double fa_n[8], fb_n[8], w_n[NW], rp_n[NW];
+    auto prefetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int e = tid + 256 * q, r = e >> 5, k = k0 + (e & 31);
+            const int fa = sfa[r], fb = sfb[r];
+            fa_n[q] = (fa >= 0) ? F[(size_t) fa * NPTS + k] : 0.0;
+            fb_n[q] = (fb >= 0) ? F[(size_t) fb * NPTS + k] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < NW; j++) {
+            const int e = tid + 256 * j;
+            if (e < (LMAX_SHELL + 1) * RK) {
+                const int k = k0 + e % RK;
+                w_n[j] = wp[k];
+                rp_n[j] = rpow[(e / RK) * NPTS + k];
+            }
+        }
+    };
+    prefetch(0);
     int lev_done = 0;
     for (int lev = 0; lev <= jb.maxlev; lev++) {
         const int kb = (lev == 0) ? 0 : (16 << lev);
@@ -910,21 +935,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
             for (int b = 0; b < 2; b++) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
         for (int k0 = kb; k0 < ke; k0 += RK) {
             // the K step's point weights once per power n, then one product per staged element
-            for (int e = tid; e < (LMAX_SHELL + 1) * RK; e += 256) {
-                const int n = e / RK, kk = e % RK, k = k0 + kk;
-                const double w = wp[k], rp = rpow[n * NPTS + k];
-                Rw[n][kk] = rp * fabs(w);
-                Rs[n][kk] = rp * w;
+#pragma unroll
+            for (int j = 0; j < NW; j++) {
+                const int e = tid + 256 * j;
+                if (e < (LMAX_SHELL + 1) * RK) {
+                    Rw[e / RK][e % RK] = rp_n[j] * fabs(w_n[j]);
+                    Rs[e / RK][e % RK] = rp_n[j] * w_n[j];
+                }
             }
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < 8; q++) {
-                const int e = tid + 256 * q, r = e >> 5, kk = e & 31, k = k0 + kk;
-                const int fa = sfa[r], fb = sfb[r];
-                As[kk][r] = (fa >= 0) ? F[(size_t) fa * NPTS + k] * Rw[sna[r]][kk] : 0.0;
-                Bs[kk][r] = (fb >= 0) ? F[(size_t) fb * NPTS + k] * Rs[snb[r]][kk] : 0.0;
+                const int e = tid + 256 * q, r = e >> 5, kk = e & 31;
+                As[kk][r] = (sfa[r] >= 0) ? fa_n[q] * Rw[sna[r]][kk] : 0.0;
+                Bs[kk][r] = (sfb[r] >= 0) ? fb_n[q] * Rs[snb[r]][kk] : 0.0;
             }
             __syncthreads();
+            if (k0 + RK < NPTS) prefetch(k0 + RK);   // in bounds: the last step ends at NPTS
 #pragma unroll
             for (int kk = 0; kk < RK; kk += 4) {
                 const double a0 = As[kk + lk][wr + li], a1 = As[kk + lk][wr + 16 + li];
@@ -944,15 +971,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
                 for (int r = 0; r < 4; r++) {
                     const unsigned bit = 1u << (a * 8 + b * 4 + r);
                     const double v = acc[a][b][r];
+                    if (done & bit) continue;
                     if (lev == 0) {
                         S[a][b][r] = v;
                     } else {
                         const double prev = S[a][b][r];
                         S[a][b][r] = 0.5 * S[a][b][r] + v;
                         const bool forced = !isnan(est[a][b][r]) && (fabs(S[a][b][r]) / fabs(est[a][b][r]) < 0.001);
-                        if (!(done & bit) && !forced && lev >= min_level && fabs(S[a][b][r] - prev) <= tol) {
+                        if (!forced && lev >= min_level && fabs(S[a][b][r] - prev) <= tol) {
                             done |= bit;
-                            res[a][b][r] = S[a][b][r];
                             if (REC) stop[a][b][r] = lev;
                         }
                     }
@@ -975,10 +1002,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
         for (int b = 0; b < 2; b++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const unsigned bit = 1u << (a * 8 + b * 4 + r);
                 const int row = t.ti * RT + wr + a * 16 + lk + 4 * r, col = t.tj * RT + wc + b * 16 + li;
                 if (row >= jb.M || col >= jb.M) continue;
-                double v = (done & bit) ? res[a][b][r] : S[a][b][r];
+                double v = S[a][b][r];
                 if (fabs(v) < jb.thr) v = 0.0;
                 if (t.ti < t.tj || row <= col) {
                     Q[jb.qoff + (long long) row * jb.M + col] = v;
