@@ -522,7 +522,7 @@ __device__ double screen_entry_wave(const DevJob &J, const DevTables &T, const D
 }
 
 template <int SCR_PROBE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_screen(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_screen(
     const TileItem *__restrict__ tiles, const QJob *__restrict__ jobs, const RowDesc *__restrict__ rows, DevJob J,
     DevTables T, double *__restrict__ scr, unsigned long long *__restrict__ counters)
 {
