@@ -107,7 +107,7 @@ int libgrpp_batch_plan_dim(const libgrpp_batch_plan_t *plan);
  * recaptured at the next execute).
  */
 typedef struct {
-    int screen_probe;         /* 4, 8 or 16 triples per entry in the screening probe */
+    int screen_probe;         /* 1, 2 or 4 triples per entry in the screening probe */
     int contract_threads;     /* 64, 128 or 256 */
     int type1_threads;        /* 64, 128 or 256 */
     int stream_layout;        /* 0: grid tables, projectors and type-1 beside the screening on a second

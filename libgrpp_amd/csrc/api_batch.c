@@ -519,7 +519,7 @@ int libgrpp_batch_plan_autotune(libgrpp_batch_plan_t *p, int repeats, libgrpp_ba
     int rc = time_builds(p, repeats, 0, &before);
     if (rc) return rc;
     /* one parameter at a time, each timed by the phase it acts on */
-    static const int probes[3] = {4, 8, 16}, threads[3] = {64, 128, 256}, layouts[2] = {0, 1};
+    static const int probes[3] = {1, 2, 4}, threads[3] = {64, 128, 256}, layouts[2] = {0, 1};
     for (int par = 0; par < 4; par++) {
         int *field = par == 0 ? &t.screen_probe : par == 1 ? &t.contract_threads : par == 2 ? &t.type1_threads
                                                                                            : &t.layout;

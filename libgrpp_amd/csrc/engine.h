@@ -46,7 +46,7 @@ typedef struct eng_plan eng_plan_t;
 
 /* launch shapes of a plan that do not change any result (the autotuner's choices) */
 typedef struct {
-    int screen_probe;       /* triples per entry in the screening probe: 4, 8 or 16 */
+    int screen_probe;       /* triples per entry in the screening probe: 1, 2 or 4 */
     int contract_threads;   /* k_contract workgroup: 64, 128 or 256 threads */
     int type1_threads;      /* k_type1 workgroup: 64, 128 or 256 threads */
     int layout;             /* 0: tables, projectors, type-1 on the side stream; 1: tables and projectors

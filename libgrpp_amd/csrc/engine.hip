@@ -440,7 +440,7 @@ __device__ __forceinline__ double screen_value(const DevTables &T, int n, int l1
            exp(-eta * r0 * r0 - aA * dA * dA - aB * dB * dB) * 0.5 * (1.0 + erf(sqrt(p) * r0));
 }
 
-// triples per entry in the screening probe (64 / PROBE entries per wave): 4, 8 or 16, picked per
+// triples per entry in the screening probe (64 / PROBE entries per wave): 1, 2 or 4, picked per
 // plan by the autotuner (the estimates do not depend on it)
 
 // Fixed point of triple q of an entry in the probe order (smallest exponents first: the triples most
@@ -575,7 +575,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     }
     // an entry with a triple that fails the fixed point has no estimate (NaN), whatever its other
     // triples give; the triples most likely to fail come first in the probe order, so a probe of
-    // the first SCR_PROBE triples of eight entries at once settles most entries of a workload
+    // the first SCR_PROBE triples of 64 / SCR_PROBE entries at once settles most entries of a workload
     auto entry = [&](int q, int &row, int &col, RowDesc &ra, RowDesc &rb, DShell &A, DShell &B, double &CA,
                      double &CB) {
         const int e = list[q];
@@ -1827,7 +1827,7 @@ struct eng_plan {
     double *d_partial_ct = nullptr;  // per-part semilocal, spin-orbit and outercore blocks (k_contract)
     int n_parts = 0;
     unsigned long long *d_counters = nullptr;
-    eng_tuning_t tune{8, 128, 256, 0};   // launch shapes and stream layout; none of them changes a result
+    eng_tuning_t tune{1, 128, 256, 0};   // launch shapes and stream layout; none of them changes a result
     hipEvent_t ev[8] = {};   // execution phases on the caller's stream: see eng_plan_execute   // execution phases on the caller's stream: see eng_plan_execute
     hipStream_t side = nullptr;   // second stream (default priority) for the radial-independent kernels
     hipStream_t cap = nullptr;    // capture stream of a graph plan (default priority)
@@ -2537,7 +2537,7 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
     }
     HIPCHK(hipEventRecord(P->ev[1], st));
     if (P->n_tiles > 0) {
-        auto screen = P->tune.screen_probe == 4 ? k_screen<4> : P->tune.screen_probe == 16 ? k_screen<16> : k_screen<8>;
+        auto screen = P->tune.screen_probe == 1 ? k_screen<1> : P->tune.screen_probe == 2 ? k_screen<2> : k_screen<4>;
         hipLaunchKernelGGL(screen, dim3(4 * P->n_tiles), dim3(64), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->J, P->T,
                            P->d_scr, P->d_counters);
     }
@@ -2731,9 +2731,9 @@ extern "C" int eng_plan_get_tuning(const eng_plan_t *P, eng_tuning_t *t)
 extern "C" int eng_plan_set_tuning(eng_plan_t *P, const eng_tuning_t *t)
 {
     auto in3 = [](int v, int a, int b, int c) { return v == a || v == b || v == c; };
-    if (!in3(t->screen_probe, 4, 8, 16) || !in3(t->contract_threads, 64, 128, 256) ||
+    if (!in3(t->screen_probe, 1, 2, 4) || !in3(t->contract_threads, 64, 128, 256) ||
         !in3(t->type1_threads, 64, 128, 256) || (t->layout != 0 && t->layout != 1)) {
-        eng_set_error("tuning: screen_probe must be 4, 8 or 16, the thread counts 64, 128 or 256, the layout 0 or 1");
+        eng_set_error("tuning: screen_probe must be 1, 2 or 4, the thread counts 64, 128 or 256, the layout 0 or 1");
         return 2;
     }
     P->tune = *t;

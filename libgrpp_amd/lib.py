@@ -170,7 +170,7 @@ class _LevelHooks:
         _check(lib().libgrpp_batch_plan_autotune(self.h, repeats, C.byref(t)), "libgrpp_batch_plan_autotune")
         return t.as_dict()
 
-    def set_tuning(self, screen_probe: int = 8, contract_threads: int = 128, type1_threads: int = 256,
+    def set_tuning(self, screen_probe: int = 1, contract_threads: int = 128, type1_threads: int = 256,
                    stream_layout: int = 0):
         t = Tuning(screen_probe, contract_threads, type1_threads, stream_layout, 0.0, 0.0)
         _check(lib().libgrpp_batch_plan_set_tuning(self.h, C.byref(t)), "libgrpp_batch_plan_set_tuning")
