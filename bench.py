@@ -42,11 +42,11 @@ def workload_config(name, sysm, ntask, nint):
             "integrals_per_step": nint, "seed": 0}
 
 
-def traffic_bytes(kernel: str):
-    """HBM bytes per launch of ``kernel`` on this workload, from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads for a plain copy of
-    known size); None when no summary is present."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_heavy_cluster.txt")
+def traffic_bytes(kernel: str, workload: str):
+    """HBM bytes per launch of ``kernel`` on ``workload``, from the committed rocprofv3 counter
+    summary of that workload (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads for a
+    plain copy of known size); None when no summary of that workload is present."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -109,7 +109,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="time the plain method only")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuning)")
     ap.add_argument("--tuning", default=None,
-                    help="fixed launch shapes screen_probe,contract_threads,type1_threads[,stream_layout] instead of autotuning")
+                    help="fixed launch shapes screen_probe,contract_threads,type1_threads[,stream_layout[,screen_split]] "
+                         "(defaults 0 and 4) instead of autotuning")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--graph", action="store_true",
                     help="also time the plain build captured once as a hipGraph and replayed (graph_leg)")
@@ -158,10 +159,10 @@ def main():
         """the plan's launch shapes picked by timing its own builds (untimed; results unchanged)"""
         if args.tuning:
             v = [int(x) for x in args.tuning.split(",")]
-            probe, ct, t1, lay = (v + [0])[:4]
-            plan.set_tuning(probe, ct, t1, lay)
+            probe, ct, t1, lay, split = (v + [0, 4][len(v) - 3:])[:5]
+            plan.set_tuning(probe, ct, t1, lay, split)
             return {"screen_probe": probe, "contract_threads": ct, "type1_threads": t1, "stream_layout": lay,
-                    "fixed": True}
+                    "screen_split": split, "fixed": True}
         if args.no_tune:
             return None
         plan.execute()
@@ -180,7 +181,8 @@ def main():
         gplan = LG.make_plan(sysm, G.BATCH_GRAPH)
         if tuning_plain:
             gplan.set_tuning(tuning_plain["screen_probe"], tuning_plain["contract_threads"],
-                             tuning_plain["type1_threads"], tuning_plain["stream_layout"])
+                             tuning_plain["type1_threads"], tuning_plain["stream_layout"],
+                             tuning_plain["screen_split"])
         for _ in range(max(1, args.warmup)):
             gplan.execute()
         torch.cuda.synchronize()
@@ -260,13 +262,13 @@ def main():
         # dominant kernel: the Shaw-Hill envelope estimates (FP64 VALU; FLOP equivalents per triple and
         # per fixed-point evaluation as counted by the engine)
         "roofline": {"bound": "mfma", "kernel": "k_screen", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_bytes("k_screen"),
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_bytes("k_screen", args.workload),
                      "screen_flops": st["screen_flops"], "screen_triples": st["screen_triples"],
                      "screen_steps": st["screen_steps"], "screen_ms": 1e3 * scr,
                      "radial_product": {"kernel": "k_radial_mfma", "achieved": rad_achieved,
                                         "frac": rad_achieved / FP64_PEAK_TFLOPS, "ms": 1e3 * rad,
                                         "flops": st["fp64_flops"], "points": st["n_radial_points"],
-                                        "tiles": st["n_tiles"], "traffic": traffic_bytes("k_radial_mfma")}},
+                                        "tiles": st["n_tiles"], "traffic": traffic_bytes("k_radial_mfma", args.workload)}},
     }
     if learned_info:
         line["learned_grid"] = learned_info

@@ -101,7 +101,7 @@ int libgrpp_batch_plan_download_flat(libgrpp_batch_plan_t *plan, double *out);  
 int libgrpp_batch_plan_dim(const libgrpp_batch_plan_t *plan);
 /*
  * Autotuner: launch shapes of a plan that change no result (bitwise), picked by timing the plan's
- * own build: the screening probe width (per-phase time of the screening), the contraction's and
+ * own build: the screening probe width and waves per tile (per-phase time of the screening), the contraction's and
  * the type-1 workgroup sizes (contraction phase, whole build) and the stream layout (whole build).  Each candidate runs `repeats`
  * eager builds and keeps its fastest; the choice stays with the plan (a captured graph is
  * recaptured at the next execute).
@@ -112,6 +112,7 @@ typedef struct {
     int type1_threads;        /* 64, 128 or 256 */
     int stream_layout;        /* 0: grid tables, projectors and type-1 beside the screening on a second
                                  stream; 1: tables and projectors ahead of it, type-1 beside it */
+    int screen_split;         /* 4, 8 or 16 screening waves per 64 x 64 radial tile */
     double seconds_before;    /* fastest build with the shapes in place before tuning */
     double seconds_after;     /* fastest build with the chosen shapes */
 } libgrpp_batch_tuning_t;

@@ -490,7 +490,7 @@ int libgrpp_batch_plan_dim(const libgrpp_batch_plan_t *p) { return p->dim; }
 
 int libgrpp_batch_plan_set_tuning(libgrpp_batch_plan_t *p, const libgrpp_batch_tuning_t *t)
 {
-    eng_tuning_t e = {t->screen_probe, t->contract_threads, t->type1_threads, t->stream_layout};
+    eng_tuning_t e = {t->screen_probe, t->contract_threads, t->type1_threads, t->stream_layout, t->screen_split};
     return eng_plan_set_tuning(p->eng, &e);
 }
 
@@ -519,11 +519,12 @@ int libgrpp_batch_plan_autotune(libgrpp_batch_plan_t *p, int repeats, libgrpp_ba
     int rc = time_builds(p, repeats, 0, &before);
     if (rc) return rc;
     /* one parameter at a time, each timed by the phase it acts on */
-    static const int probes[3] = {1, 2, 4}, threads[3] = {64, 128, 256}, layouts[2] = {0, 1};
-    for (int par = 0; par < 4; par++) {
+    static const int probes[3] = {1, 2, 4}, splits[3] = {4, 8, 16}, threads[3] = {64, 128, 256}, layouts[2] = {0, 1};
+    /* par: 0 probe width, 1 contraction threads, 2 type-1 threads, 3 stream layout, 4 screening split */
+    for (int par = 0; par < 5; par++) {
         int *field = par == 0 ? &t.screen_probe : par == 1 ? &t.contract_threads : par == 2 ? &t.type1_threads
-                                                                                           : &t.layout;
-        const int *cand = par == 0 ? probes : par == 3 ? layouts : threads;
+                   : par == 3 ? &t.layout : &t.screen_split;
+        const int *cand = par == 0 ? probes : par == 3 ? layouts : par == 4 ? splits : threads;
         const int ncand = par == 3 ? 2 : 3;
         int best_v = *field;
         double best_t = 1e300;
@@ -531,7 +532,7 @@ int libgrpp_batch_plan_autotune(libgrpp_batch_plan_t *p, int repeats, libgrpp_ba
             *field = cand[c];
             if ((rc = eng_plan_set_tuning(p->eng, &t))) return rc;
             double v;
-            if ((rc = time_builds(p, repeats, par == 0 ? 1 : par == 1 ? 2 : 0, &v))) return rc;
+            if ((rc = time_builds(p, repeats, par == 0 || par == 4 ? 1 : par == 1 ? 2 : 0, &v))) return rc;
             if (v < best_t) {
                 best_t = v;
                 best_v = cand[c];
@@ -547,6 +548,7 @@ int libgrpp_batch_plan_autotune(libgrpp_batch_plan_t *p, int repeats, libgrpp_ba
         chosen->contract_threads = t.contract_threads;
         chosen->type1_threads = t.type1_threads;
         chosen->stream_layout = t.layout;
+        chosen->screen_split = t.screen_split;
         chosen->seconds_before = before;
         chosen->seconds_after = after;
     }

@@ -521,35 +521,38 @@ __device__ double screen_entry_wave(const DevJob &J, const DevTables &T, const D
     return est;
 }
 
-template <int SCR_PROBE>
+// SCR_SPLIT waves per 64 x 64 tile (4, 8 or 16, picked per plan): a wave takes the rows sub,
+// sub + SCR_SPLIT, ... of its tile.  The estimates do not depend on it; finer splits shorten the tail
+// of a launch with few tiles, whose waves differ widely in how many entries their probe leaves.
+template <int SCR_PROBE, int SCR_SPLIT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_screen(
     const TileItem *__restrict__ tiles, const QJob *__restrict__ jobs, const RowDesc *__restrict__ rows, DevJob J,
     DevTables T, double *__restrict__ scr, unsigned long long *__restrict__ counters)
 {
-    // one wave per quarter tile (rows sub, sub+4, ...).  The wave first lists the entries it must
+    // one wave per 1/SCR_SPLIT of a tile (rows sub, sub + SCR_SPLIT, ...).  The wave first lists the entries it must
     // estimate -- inside the table, upper triangle, canonical (lambda1, lambda2, N) split -- with the
     // full-cost ones first and the ones that vanish analytically (K_l(0) = 0 for l >= 1 on a shell
     // sitting on the centre) last, so lanes run entries of like cost side by side.
-    __shared__ int list[RT * RT / 4];
-    __shared__ unsigned char settled[RT * RT / 4];   // entries whose probe found a failing triple
+    __shared__ int list[RT * RT / SCR_SPLIT];
+    __shared__ unsigned char settled[RT * RT / SCR_SPLIT];   // entries whose probe found a failing triple
     __shared__ int n_full, n_zero;
-    const TileItem t = tiles[blockIdx.x >> 2];
+    const TileItem t = tiles[blockIdx.x / SCR_SPLIT];
     const QJob jb = jobs[t.job];
     const RowDesc *R = rows + jb.rows;
     const DCenter c = J.ctr[jb.center];
     const DPot P = J.pot[jb.potential];
-    const int sub = blockIdx.x & 3;
+    const int sub = blockIdx.x % SCR_SPLIT;
     if (threadIdx.x == 0) {
         n_full = 0;
         n_zero = 0;
     }
     __syncthreads();
-    constexpr int NE = RT * RT / 4;
+    constexpr int NE = RT * RT / SCR_SPLIT;
     // the quarter tiles cover the table in its logical row order (shell, n, lambda), whatever the
     // physical order of the rows, so the work of a wave does not depend on a row permutation
     auto phys = [&](int l) { return J.rowpos[jb.rows + l]; };
     for (int e = threadIdx.x; e < NE; e += blockDim.x) {
-        const int lrow = t.ti * RT + (e / RT) * 4 + sub, lcol = t.tj * RT + e % RT;
+        const int lrow = t.ti * RT + (e / RT) * SCR_SPLIT + sub, lcol = t.tj * RT + e % RT;
         if (lrow >= jb.M || lcol >= jb.M) continue;
         if (t.ti == t.tj && lrow > lcol) continue;
         const RowDesc ra = R[phys(lrow)], rb = R[phys(lcol)];
@@ -569,7 +572,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     // analytically vanishing entries: every triple is zero, so the estimate is 0 (screened out)
     for (int q = threadIdx.x; q < nz; q += blockDim.x) {
         const int e = list[NE - nz + q];
-        const int row = phys(t.ti * RT + (e / RT) * 4 + sub), col = phys(t.tj * RT + e % RT);
+        const int row = phys(t.ti * RT + (e / RT) * SCR_SPLIT + sub), col = phys(t.tj * RT + e % RT);
         scr[jb.qoff + (long long) row * jb.M + col] = 0.0;
         scr[jb.qoff + (long long) col * jb.M + row] = 0.0;
     }
@@ -579,7 +582,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     auto entry = [&](int q, int &row, int &col, RowDesc &ra, RowDesc &rb, DShell &A, DShell &B, double &CA,
                      double &CB) {
         const int e = list[q];
-        row = phys(t.ti * RT + (e / RT) * 4 + sub);
+        row = phys(t.ti * RT + (e / RT) * SCR_SPLIT + sub);
         col = phys(t.tj * RT + e % RT);
         ra = R[row];
         rb = R[col];
@@ -1827,7 +1830,7 @@ struct eng_plan {
     double *d_partial_ct = nullptr;  // per-part semilocal, spin-orbit and outercore blocks (k_contract)
     int n_parts = 0;
     unsigned long long *d_counters = nullptr;
-    eng_tuning_t tune{1, 128, 256, 0};   // launch shapes and stream layout; none of them changes a result
+    eng_tuning_t tune{1, 128, 256, 0, 4};   // launch shapes and stream layout; none of them changes a result
     hipEvent_t ev[8] = {};   // execution phases on the caller's stream: see eng_plan_execute   // execution phases on the caller's stream: see eng_plan_execute
     hipStream_t side = nullptr;   // second stream (default priority) for the radial-independent kernels
     hipStream_t cap = nullptr;    // capture stream of a graph plan (default priority)
@@ -2365,6 +2368,9 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     P->n_tab = (int) tab.size();
     P->n_wpots = (int) wpots.size();
     P->n_tiles = (int) tiles.size();
+    // default screening split: 16 waves per tile when four per tile would not give the chip's
+    // 4096 wave slots (256 CUs x 4 SIMDs x 4 waves) about eight waves each, else 4
+    P->tune.screen_split = 4LL * P->n_tiles < 8 * 4096 ? 16 : 4;
     P->n_proj = (int) proj.size();
     P->n_ocv = (int) ocv.size();
     P->n_delta_centers = (int) delta_centers.size();
@@ -2537,8 +2543,11 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
     }
     HIPCHK(hipEventRecord(P->ev[1], st));
     if (P->n_tiles > 0) {
-        auto screen = P->tune.screen_probe == 1 ? k_screen<1> : P->tune.screen_probe == 2 ? k_screen<2> : k_screen<4>;
-        hipLaunchKernelGGL(screen, dim3(4 * P->n_tiles), dim3(64), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->J, P->T,
+        const int sp = P->tune.screen_probe, ss = P->tune.screen_split;
+        auto screen = ss == 16 ? (sp == 1 ? k_screen<1, 16> : sp == 2 ? k_screen<2, 16> : k_screen<4, 16>)
+                    : ss == 8  ? (sp == 1 ? k_screen<1, 8> : sp == 2 ? k_screen<2, 8> : k_screen<4, 8>)
+                               : (sp == 1 ? k_screen<1, 4> : sp == 2 ? k_screen<2, 4> : k_screen<4, 4>);
+        hipLaunchKernelGGL(screen, dim3(ss * P->n_tiles), dim3(64), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->J, P->T,
                            P->d_scr, P->d_counters);
     }
     HIPCHK(hipStreamWaitEvent(st, P->ev[5], 0));   // grid tables of the side stream
@@ -2731,9 +2740,10 @@ extern "C" int eng_plan_get_tuning(const eng_plan_t *P, eng_tuning_t *t)
 extern "C" int eng_plan_set_tuning(eng_plan_t *P, const eng_tuning_t *t)
 {
     auto in3 = [](int v, int a, int b, int c) { return v == a || v == b || v == c; };
-    if (!in3(t->screen_probe, 1, 2, 4) || !in3(t->contract_threads, 64, 128, 256) ||
+    if (!in3(t->screen_probe, 1, 2, 4) || !in3(t->screen_split, 4, 8, 16) || !in3(t->contract_threads, 64, 128, 256) ||
         !in3(t->type1_threads, 64, 128, 256) || (t->layout != 0 && t->layout != 1)) {
-        eng_set_error("tuning: screen_probe must be 1, 2 or 4, the thread counts 64, 128 or 256, the layout 0 or 1");
+        eng_set_error("tuning: screen_probe must be 1, 2 or 4, screen_split 4, 8 or 16, the thread counts 64, 128 or "
+                      "256, the layout 0 or 1");
         return 2;
     }
     P->tune = *t;

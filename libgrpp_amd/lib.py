@@ -156,7 +156,8 @@ def batch_gradients(sysm: System, flags: int = 0):
 class Tuning(C.Structure):
     """libgrpp_batch_tuning_t"""
     _fields_ = [("screen_probe", C.c_int), ("contract_threads", C.c_int), ("type1_threads", C.c_int),
-                ("stream_layout", C.c_int), ("seconds_before", C.c_double), ("seconds_after", C.c_double)]
+                ("stream_layout", C.c_int), ("screen_split", C.c_int), ("seconds_before", C.c_double),
+                ("seconds_after", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -171,8 +172,8 @@ class _LevelHooks:
         return t.as_dict()
 
     def set_tuning(self, screen_probe: int = 1, contract_threads: int = 128, type1_threads: int = 256,
-                   stream_layout: int = 0):
-        t = Tuning(screen_probe, contract_threads, type1_threads, stream_layout, 0.0, 0.0)
+                   stream_layout: int = 0, screen_split: int = 4):
+        t = Tuning(screen_probe, contract_threads, type1_threads, stream_layout, screen_split, 0.0, 0.0)
         _check(lib().libgrpp_batch_plan_set_tuning(self.h, C.byref(t)), "libgrpp_batch_plan_set_tuning")
 
     def features(self) -> np.ndarray:

@@ -1,6 +1,6 @@
 """The autotuner changes launch shapes only: every shape it may pick gives bitwise the matrices of
-the default shapes (screening probe width 1/2/4, contraction and type-1 workgroups of 64/128/256
-threads, either stream layout), and a tuned plan, eager or graph-replayed, still does."""
+the default shapes (screening probe width 1/2/4 and 4/8/16 waves per tile, contraction and type-1
+workgroups of 64/128/256 threads, either stream layout), and a tuned plan, eager or graph-replayed, still does."""
 import numpy as np
 import pytest
 
@@ -16,11 +16,12 @@ def test_every_launch_shape_gives_the_same_matrices(workload):
     plan = G.Plan(sysm)
     plan.execute()
     want = plan.download()
-    for probe, ct, t1, lay in ((4, 64, 64, 1), (2, 256, 128, 0), (1, 128, 256, 1), (4, 256, 64, 0), (2, 64, 256, 1)):
-        plan.set_tuning(probe, ct, t1, lay)
+    for shape in ((4, 64, 64, 1, 16), (2, 256, 128, 0, 8), (1, 128, 256, 1, 4), (4, 256, 64, 0, 16),
+                  (2, 64, 256, 1, 8), (1, 128, 128, 0, 16), (1, 128, 128, 0, 4)):
+        plan.set_tuning(*shape)
         plan.execute()
         got = plan.download()
-        assert all(np.array_equal(a, b) for a, b in zip(want, got)), (probe, ct, t1, lay)
+        assert all(np.array_equal(a, b) for a, b in zip(want, got)), shape
     plan.close()
 
 
@@ -36,10 +37,13 @@ def test_autotuned_graph_plan_keeps_results():
     t = plan.autotune(repeats=1)         # eager timing runs; the graph is recaptured afterwards
     print("tuned:", t, flush=True)
     assert t["screen_probe"] in (1, 2, 4) and t["contract_threads"] in (64, 128, 256)
+    assert t["screen_split"] in (4, 8, 16)
     assert t["seconds_after"] > 0.0
     for _ in range(2):
         plan.execute()
         assert all(np.array_equal(a, b) for a, b in zip(want, plan.download()))
     with pytest.raises(RuntimeError, match="tuning"):
         plan.set_tuning(5, 128, 256, 0)
+    with pytest.raises(RuntimeError, match="tuning"):
+        plan.set_tuning(1, 128, 256, 0, 2)
     plan.close()
