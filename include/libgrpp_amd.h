@@ -112,7 +112,9 @@ typedef struct {
     int type1_threads;        /* 64, 128 or 256 */
     int stream_layout;        /* 0: grid tables, projectors and type-1 beside the screening on a second
                                  stream; 1: tables and projectors ahead of it, type-1 beside it */
-    int screen_split;         /* 4, 8 or 16 screening waves per 64 x 64 radial tile */
+    int screen_split;         /* 4, 8 or 16 screening waves per 64 x 64 radial tile; 0 in
+                                 libgrpp_batch_plan_set_tuning keeps the plan's current split (the
+                                 plan's default: 16 when it has fewer than 8192 radial tiles, else 4) */
     double seconds_before;    /* fastest build with the shapes in place before tuning */
     double seconds_after;     /* fastest build with the chosen shapes */
 } libgrpp_batch_tuning_t;

@@ -491,6 +491,11 @@ int libgrpp_batch_plan_dim(const libgrpp_batch_plan_t *p) { return p->dim; }
 int libgrpp_batch_plan_set_tuning(libgrpp_batch_plan_t *p, const libgrpp_batch_tuning_t *t)
 {
     eng_tuning_t e = {t->screen_probe, t->contract_threads, t->type1_threads, t->stream_layout, t->screen_split};
+    if (e.screen_split == 0) {   /* 0 keeps the plan's current split (a caller that predates the field) */
+        eng_tuning_t cur;
+        eng_plan_get_tuning(p->eng, &cur);
+        e.screen_split = cur.screen_split;
+    }
     return eng_plan_set_tuning(p->eng, &e);
 }
 

@@ -172,7 +172,8 @@ class _LevelHooks:
         return t.as_dict()
 
     def set_tuning(self, screen_probe: int = 1, contract_threads: int = 128, type1_threads: int = 256,
-                   stream_layout: int = 0, screen_split: int = 4):
+                   stream_layout: int = 0, screen_split: int = 0):
+        """fixed launch shapes (screen_split 0 keeps the plan's current split)"""
         t = Tuning(screen_probe, contract_threads, type1_threads, stream_layout, screen_split, 0.0, 0.0)
         _check(lib().libgrpp_batch_plan_set_tuning(self.h, C.byref(t)), "libgrpp_batch_plan_set_tuning")
 

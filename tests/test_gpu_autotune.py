@@ -17,7 +17,7 @@ def test_every_launch_shape_gives_the_same_matrices(workload):
     plan.execute()
     want = plan.download()
     for shape in ((4, 64, 64, 1, 16), (2, 256, 128, 0, 8), (1, 128, 256, 1, 4), (4, 256, 64, 0, 16),
-                  (2, 64, 256, 1, 8), (1, 128, 128, 0, 16), (1, 128, 128, 0, 4)):
+                  (2, 64, 256, 1, 8), (1, 128, 128, 0, 16), (1, 128, 128, 0, 4), (1, 128, 128, 0, 0)):
         plan.set_tuning(*shape)
         plan.execute()
         got = plan.download()
@@ -44,6 +44,8 @@ def test_autotuned_graph_plan_keeps_results():
         assert all(np.array_equal(a, b) for a, b in zip(want, plan.download()))
     with pytest.raises(RuntimeError, match="tuning"):
         plan.set_tuning(5, 128, 256, 0)
+    with pytest.raises(RuntimeError, match="tuning"):
+        plan.set_tuning(1, 128, 256, 0, 2)
     with pytest.raises(RuntimeError, match="tuning"):
         plan.set_tuning(1, 128, 256, 0, 2)
     plan.close()
