@@ -525,7 +525,7 @@ __device__ double screen_entry_wave(const DevJob &J, const DevTables &T, const D
 // sub + SCR_SPLIT, ... of its tile.  The estimates do not depend on it; finer splits shorten the tail
 // of a launch with few tiles, whose waves differ widely in how many entries their probe leaves.
 template <int SCR_PROBE, int SCR_SPLIT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_screen(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_screen(
     const TileItem *__restrict__ tiles, const QJob *__restrict__ jobs, const RowDesc *__restrict__ rows, DevJob J,
     DevTables T, double *__restrict__ scr, unsigned long long *__restrict__ counters)
 {
@@ -2369,7 +2369,7 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     P->n_wpots = (int) wpots.size();
     P->n_tiles = (int) tiles.size();
     // default screening split: 16 waves per tile when four per tile would not give the chip's
-    // 4096 wave slots (256 CUs x 4 SIMDs x 4 waves) about eight waves each, else 4
+    // 5120 screening wave slots (256 CUs x 4 SIMDs x 5 waves) about six waves each, else 4
     P->tune.screen_split = 4LL * P->n_tiles < 8 * 4096 ? 16 : 4;
     P->n_proj = (int) proj.size();
     P->n_ocv = (int) ocv.size();
