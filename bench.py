@@ -213,7 +213,12 @@ def main():
         LG.apply(plan, net, bf16=True)
         torch.cuda.synchronize()
         t_pred = time.perf_counter() - t0
-        tuning = tune(plan)
+        # the plain plan's tuned launch shapes, so that the two timings differ only in the row order
+        tuning = tuning_plain
+        if tuning_plain:
+            plan.set_tuning(tuning_plain["screen_probe"], tuning_plain["contract_threads"],
+                            tuning_plain["type1_threads"], tuning_plain["stream_layout"],
+                            tuning_plain["screen_split"])
         wall, st, mean = timed(plan)
         out = plan.download()
         plan.close()
@@ -230,7 +235,7 @@ def main():
                         "radial_ms_learned": 1e3 * mean["radial_seconds"],
                         "max_abs_diff_vs_plain": diff, "training_rows": int(X.shape[0]),
                         "training_loss": loss, "training_s": t_train, "inference_and_reorder_ms": 1e3 * t_pred,
-                        "autotune_plain": tuning_plain, "autotune_learned": tuning}
+                        "launch_shapes": "the plain plan's tuned shapes, used for both plans"}
 
     ntask = st["n_tasks"]
     nint = st["n_integrals"]
@@ -272,7 +277,7 @@ def main():
     }
     if learned_info:
         line["learned_grid"] = learned_info
-    line["autotune"] = learned_info["autotune_learned"] if learned_info else tuning_plain
+    line["autotune"] = tuning_plain
     if graph_info:
         line["graph_leg"] = graph_info
     if not args.no_cpu:
