@@ -1347,10 +1347,13 @@ __global__ __launch_bounds__(256) void k_type1(const PartItem *__restrict__ part
 
 // outercore vectors: [S (ncA x (2L+1)), U (ncA x (2L+1))] per (centre, shell, OC entry)
 __global__ __launch_bounds__(256) void k_ocv(const OcvItem *__restrict__ items, DevJob J, DevTables T,
-                                             double *__restrict__ OCV)
+                                             double *__restrict__ OCV, int scratch_len, int nel_max)
 {
-    __shared__ double scratch[MMD_SCRATCH];
-    __shared__ double bs[MAX_OCB], bu[MAX_OCB];
+    // dynamic LDS sized by the plan (ocv_scratch, ocv_nel) instead of the upper bounds (96 KiB of
+    // E / R tables, MAX_OCB elements), so that several workgroups fit on a CU beside the kernels of
+    // the other stream: E / R tables, then the overlap and potential blocks
+    extern __shared__ double ocv_lds[];
+    double *scratch = ocv_lds, *bs = ocv_lds + scratch_len, *bu = bs + nel_max;
     __shared__ int one_pow;
     __shared__ double one_coef, zero_zeta;
     const OcvItem it = items[blockIdx.x];
@@ -1373,9 +1376,9 @@ __global__ __launch_bounds__(256) void k_ocv(const OcvItem *__restrict__ items, 
     const MmdShell O = {so.L, so.nprim, so.poff, c.x, c.y, c.z};
     // overlap <A|phi>: the n = 2 kernel with a zero exponent (overlap.c:39-138 gives the same integral)
     mmd_accumulate(A, O, c.x, c.y, c.z, &one_pow, &one_coef, &zero_zeta, 0, 1, J.alpha, J.craw, bs, scratch,
-                   MMD_SCRATCH, T.boys);
+                   scratch_len, T.boys);
     mmd_accumulate(A, O, c.x, c.y, c.z, J.tpow, J.tcoef, J.tzeta, p.toff, p.toff + p.nterm, J.alpha, J.craw, bu,
-                   scratch, MMD_SCRATCH, T.boys);
+                   scratch, scratch_len, T.boys);
     __syncthreads();
     const double *slm = T.slm + T.slm_off[L];
     for (int e = threadIdx.x; e < ncA * nm; e += blockDim.x) {
@@ -1853,6 +1856,7 @@ struct eng_plan {
     int nm_max = 1;
     int ct_qs = 1, ct_ts = 1;   // k_contract LDS doubles
     int t1_scratch = 64, t1_acc = 1;   // k_type1 LDS doubles
+    int ocv_scratch = 64, ocv_nel = 1;  // k_ocv LDS doubles: E / R tables, each of its two blocks
     // radial rows: logical order per slot (shell, n, lambda), their physical order in the tables,
     // per-row features for the level predictor, and the recorded stopping levels
     std::vector<RowDesc> rows_logical;
@@ -2212,6 +2216,20 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
                 ocv_tab[(size_t) o * nshell + A] = ocvlen;
                 ocvlen += 2LL * ncart(sh[A].L) * (2 * sh[oce[o].shell].L + 1);
                 ocv.push_back(it);
+                // LDS of k_ocv that batches the primitive pairs exactly as the 96 KiB bound would
+                // (mmd_accumulate takes min(64, scratch / esz) pairs per batch for r^0 terms, one
+                // pair with two R/G layers otherwise), so the sums are the same
+                const int la = sh[A].L, lo = sh[oce[o].shell].L, N = la + lo;
+                const long long esz = 3LL * (la + 1) * (lo + 1) * (N + 1);
+                const long long ssz = (long long) (N + 1) * (N + 2) * (N + 3) / 6;
+                const long long npair = (long long) sh[A].nprim * sh[oce[o].shell].nprim;
+                const long long bmax = std::min<long long>(64, MMD_SCRATCH / esz);
+                long long need = std::max<long long>(1, std::min(npair, bmax)) * esz;
+                const DPot &op = pots[oce[o].pot];
+                for (int k = op.toff; k < op.toff + op.nterm; k++)
+                    if (tpow[k] != 2) need = std::max(need, esz + 2 * ssz + 64);
+                P->ocv_scratch = (int) std::max<long long>(P->ocv_scratch, std::min<long long>(need, MMD_SCRATCH));
+                P->ocv_nel = std::max(P->ocv_nel, ncart(la) * ncart(lo));
             }
     }
     // work items
@@ -2531,7 +2549,9 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
     if (P->n_delta_centers > 0)
         hipLaunchKernelGGL(k_delta, dim3(P->n_delta_centers), dim3(64), 0, sd, P->d_delta_centers,
                            P->n_delta_centers, P->J, P->d_delta);
-    if (P->n_ocv > 0) hipLaunchKernelGGL(k_ocv, dim3(P->n_ocv), dim3(256), 0, sd, P->d_ocvi, P->J, P->T, P->d_OCV);
+    if (P->n_ocv > 0)
+        hipLaunchKernelGGL(k_ocv, dim3(P->n_ocv), dim3(256), (size_t) (P->ocv_scratch + 2 * P->ocv_nel) * sizeof(double),
+                           sd, P->d_ocvi, P->J, P->T, P->d_OCV, P->ocv_scratch, P->ocv_nel);
     HIPCHK(hipEventRecord(P->evs[1], sd));
     if (!front) {
         HIPCHK(hipEventRecord(P->evt1, sd));
@@ -2590,6 +2610,9 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
     hipStream_t st = (hipStream_t) stream_;
     // dynamic LDS beyond the 64 KiB default (up to the 160 KiB of a gfx950 CU) must be opted into;
     // the attribute belongs to the kernel, not the plan, so each execution sets its plan's sizes
+    if (P->n_ocv > 0)
+        HIPCHK(hipFuncSetAttribute((const void *) k_ocv, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (P->ocv_scratch + 2 * P->ocv_nel) * (int) sizeof(double)));
     if (P->n_parts > 0) {
         HIPCHK(hipFuncSetAttribute((const void *) k_type1, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (P->t1_scratch + P->t1_acc) * (int) sizeof(double)));
