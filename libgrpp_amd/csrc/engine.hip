@@ -1718,6 +1718,27 @@ static int upload(const std::vector<T> &v, T **out, std::vector<void *> &keep)
     return 0;
 }
 
+typedef void (*contract_fn)(const PartItem *, const WorkItem *, DevJob, DevTables, double *, int, int, int, int);
+static contract_fn contract_kernel(int threads);
+
+// Dynamic LDS beyond the 64 KiB default has to be opted into per kernel, and that attribute belongs
+// to the kernel for the whole process, not to one plan.  It is raised once per device, when the
+// device tables are first set up (under g_tab_mutex), to everything a gfx950 CU offers beside the
+// kernel's static LDS; each launch still asks only for its plan's size, which alone sets occupancy.
+// Setting it per plan instead would let a thread lower it under another thread's larger launch.
+static int raise_lds_limits(void)
+{
+    const int cu_lds = 160 * 1024;
+    const void *fns[] = {(const void *) k_ocv, (const void *) k_type1, (const void *) contract_kernel(64),
+                         (const void *) contract_kernel(128), (const void *) contract_kernel(256)};
+    for (const void *f : fns) {
+        hipFuncAttributes a{};
+        HIPCHK(hipFuncGetAttributes(&a, f));
+        HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, cu_lds - (int) a.sharedSizeBytes));
+    }
+    return 0;
+}
+
 static int device_tables(int Lmax, int nmax, DevTables *out)
 {
     std::lock_guard<std::mutex> lk(g_tab_mutex);
@@ -1725,6 +1746,7 @@ static int device_tables(int Lmax, int nmax, DevTables *out)
     HIPCHK(hipGetDevice(&dev));
     DeviceTables &D = g_dev_tables[dev];
     if (!D.ok) {
+        if (raise_lds_limits()) return 1;
         const HostTables &H = host_tables();
         double *p;
         int *ip;
@@ -2496,7 +2518,6 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
 
 static int eager_phase_times(eng_plan_t *P);
 
-typedef void (*contract_fn)(const PartItem *, const WorkItem *, DevJob, DevTables, double *, int, int, int, int);
 static contract_fn contract_kernel(int threads)
 {
     return threads == 64 ? k_contract<64> : threads == 256 ? k_contract<256> : k_contract<128>;
@@ -2608,17 +2629,6 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
 extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
 {
     hipStream_t st = (hipStream_t) stream_;
-    // dynamic LDS beyond the 64 KiB default (up to the 160 KiB of a gfx950 CU) must be opted into;
-    // the attribute belongs to the kernel, not the plan, so each execution sets its plan's sizes
-    if (P->n_ocv > 0)
-        HIPCHK(hipFuncSetAttribute((const void *) k_ocv, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (P->ocv_scratch + 2 * P->ocv_nel) * (int) sizeof(double)));
-    if (P->n_parts > 0) {
-        HIPCHK(hipFuncSetAttribute((const void *) k_type1, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (P->t1_scratch + P->t1_acc) * (int) sizeof(double)));
-        HIPCHK(hipFuncSetAttribute((const void *) contract_kernel(P->tune.contract_threads),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (P->ct_qs + P->ct_ts) * (int) sizeof(double)));
-    }
     if (P->flags & ENG_F_GRAPH) {
         // the build is captured once, on a plan-owned stream, and replayed on the caller's stream;
         // kernel arguments are device pointers and sizes fixed at plan creation, so a replay sees

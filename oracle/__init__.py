@@ -22,6 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORC_SO = os.path.join(HERE, "liborc.so")
 ORC_NOFMA_SO = os.path.join(HERE, "liborc_nofma.so")
 REF_SO = os.path.join(HERE, "_ref", "libgrpp_ref.so")
+REF_NOFMA_SO = os.path.join(HERE, "_ref", "libgrpp_ref_nofma.so")
 
 ORC_NAMES = dict(new_shell="orc_new_shell", delete_shell="orc_delete_shell", new_potential="orc_new_potential",
                  delete_potential="orc_delete_potential", new_grpp="orc_new_grpp", set_local="orc_grpp_set_local",
@@ -132,6 +133,15 @@ def reference() -> Optional[CpuLib]:
     if "ref" not in _cache:
         _cache["ref"] = CpuLib(REF_SO, "reference") if os.path.exists(REF_SO) else None
     return _cache["ref"]
+
+
+def reference_nofma() -> Optional[CpuLib]:
+    """The genuine reference sources compiled with floating-point contraction off: the reference's
+    own second rounding of the same algorithm.  |reference() - reference_nofma()| per element is the
+    noise floor of an ill-conditioned input."""
+    if "ref_nofma" not in _cache:
+        _cache["ref_nofma"] = CpuLib(REF_NOFMA_SO, "reference") if os.path.exists(REF_NOFMA_SO) else None
+    return _cache["ref_nofma"]
 
 
 # ---------------------------------------------------------------- per-call entry points

@@ -12,7 +12,12 @@ the reference sources; the fixtures are data only).
 * ``h2o_pergrad_reference.npz`` -- libgrpp_full_grpp_integrals_gradient of the genuine library for
   a few (A, B, C, D) tasks of the H2O case (covering the ACB / ACA / ACC / CCB branches).
 
-    python tests/golden/make_golden.py
+* ``heavy_cluster_reference.npz`` / ``ragged_reference.npz`` -- full-GRPP blocks of sampled
+  ordered shell pairs (all centres) of the heavy_cluster (seed 0) and 200-molecule ragged (seed 0)
+  workloads, from the genuine reference and from the same sources compiled with floating-point
+  contraction off (the per-element spread of the two is the reference's own rounding floor).
+
+    python tests/golden/make_golden.py [fixture ...]
 """
 import ctypes as C
 import os
@@ -124,8 +129,65 @@ def h2o_pergrad_reference():
                         values=np.concatenate(out), lengths=np.array([len(o) for o in out]))
 
 
+def _sampled_blocks(systems, picks):
+    """Full-GRPP blocks of the genuine reference (FMA build) and of the same sources compiled with
+    contraction off, for sampled ordered shell pairs: picks = [(molecule, [pair index i*ns + j])].
+    Returns flat arrays: molecule, pair, offset of each block in the flattened 4-operator values."""
+    import oracle
+    ref, nof = oracle.reference(), oracle.reference_nofma()
+    if ref is None or nof is None:
+        raise SystemExit("oracle/_ref not built: run make -C oracle ref")
+    mol, pair, offs, vals, valn = [], [], [0], [], []
+    for m, pairs in picks:
+        sysm = systems[m]
+        pairs = np.sort(np.asarray(pairs, dtype=np.int32))
+        R = ref.matrices(sysm, nthreads=os.cpu_count(), pairs=pairs)
+        N = nof.matrices(sysm, nthreads=os.cpu_count(), pairs=pairs)
+        off = sysm.offsets() + [sysm.dim]
+        ns = len(sysm.shells)
+        for p in pairs:
+            i, j = divmod(int(p), ns)
+            blk = lambda X: np.array([X[k][off[i]:off[i + 1], off[j]:off[j + 1]] for k in range(4)]).ravel()
+            mol.append(m)
+            pair.append(int(p))
+            vals.append(blk(R))
+            valn.append(blk(N))
+            offs.append(offs[-1] + vals[-1].size)
+    return dict(molecule=np.array(mol, np.int32), pair=np.array(pair, np.int32), offset=np.array(offs, np.int64),
+                ref=np.concatenate(vals), ref_nofma=np.concatenate(valn))
+
+
+def heavy_cluster_reference(npair: int = 128):
+    """heavy_cluster (8 atoms, seed 0, 112 shells, 8 centres): a fixed random sample of ordered
+    shell pairs, every centre, through the genuine reference and its contraction-off twin"""
+    from libgrpp_amd import configs
+    sysm = configs.heavy_cluster()
+    ns = len(sysm.shells)
+    pairs = np.random.default_rng(2026).permutation(ns * ns)[:npair]
+    d = _sampled_blocks([sysm], [(0, pairs)])
+    np.savez_compressed(os.path.join(HERE, "heavy_cluster_reference.npz"), **d)
+
+
+def ragged_reference(nmol: int = 8, npair: int = 12):
+    """the 200-molecule ragged workload (seed 0): nmol molecules picked at random, npair random
+    ordered shell pairs of each with all of the molecule's centres"""
+    from libgrpp_amd import configs
+    mols = configs.ragged_molecules()
+    rng = np.random.default_rng(2027)
+    pick = sorted(int(x) for x in rng.choice(len(mols), nmol, replace=False))
+    picks = []
+    for m in pick:
+        ns = len(mols[m].shells)
+        picks.append((m, rng.permutation(ns * ns)[:npair]))
+    d = _sampled_blocks(mols, picks)
+    np.savez_compressed(os.path.join(HERE, "ragged_reference.npz"), **d)
+
+
+FIXTURES = {"tables": tables_sample, "small_pair": small_pair_reference, "h2o_pergrad": h2o_pergrad_reference,
+            "heavy_cluster": heavy_cluster_reference, "ragged": ragged_reference}
+
+
 if __name__ == "__main__":
-    tables_sample()
-    small_pair_reference()
-    h2o_pergrad_reference()
-    print("written")
+    for name in (sys.argv[1:] or list(FIXTURES)):
+        FIXTURES[name]()
+        print("written", name, flush=True)
