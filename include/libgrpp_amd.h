@@ -22,8 +22,10 @@ typedef struct libgrpp_batch_plan libgrpp_batch_plan_t;
 /* flags */
 enum {
     LIBGRPP_BATCH_DEFAULT = 0,
-    LIBGRPP_BATCH_LEARNED_GRID = 1,   /* predicted fixed radial grids + embedded-rule check + adaptive fallback */
-    LIBGRPP_BATCH_GRADIENT = 2,       /* plan also evaluates nuclear gradients (see ..._gradients) */
+    LIBGRPP_BATCH_LEARNED_GRID = 1,   /* reserved, refused: a plan takes the level predictor's output through
+                                         libgrpp_batch_plan_set_predictions */
+    LIBGRPP_BATCH_GRADIENT = 2,       /* reserved, refused: gradients are built by
+                                         libgrpp_batch_gradients_plan_create / libgrpp_batch_full_grpp_gradients */
     LIBGRPP_BATCH_NO_SYMMETRY = 4,    /* evaluate every ordered pair instead of mirroring B,A from A,B */
     LIBGRPP_BATCH_RECORD_LEVELS = 8,  /* plan records each radial entry's stopping level (training labels) */
     LIBGRPP_BATCH_GRAPH = 16          /* execute replays one hipGraph of the whole build, captured by the
@@ -114,7 +116,9 @@ typedef struct {
                                  stream; 1: tables and projectors ahead of it, type-1 beside it */
     int screen_split;         /* 4, 8 or 16 screening waves per 64 x 64 radial tile; 0 in
                                  libgrpp_batch_plan_set_tuning keeps the plan's current split (the
-                                 plan's default: 16 when it has fewer than 8192 radial tiles, else 4) */
+                                 plan's default: 16 when it has fewer than 8192 radial tiles, else 4).
+                                 The struct is 40 bytes (this field at offset 16); callers compiled
+                                 against an older 32-byte layout must be rebuilt. */
     double seconds_before;    /* fastest build with the shapes in place before tuning */
     double seconds_after;     /* fastest build with the chosen shapes */
 } libgrpp_batch_tuning_t;

@@ -305,6 +305,20 @@ static int check_canonical(int n, libgrpp_shell_t *const *shells)
     return 1;
 }
 
+/* Flags a plan accepts.  LIBGRPP_BATCH_LEARNED_GRID and LIBGRPP_BATCH_GRADIENT are refused with
+ * an error rather than ignored: the level predictor reaches a plan through
+ * libgrpp_batch_plan_set_predictions, and gradients through libgrpp_batch_gradients_plan_create. */
+static int check_flags(const char *who, int flags)
+{
+    const int known = LIBGRPP_BATCH_NO_SYMMETRY | LIBGRPP_BATCH_RECORD_LEVELS | LIBGRPP_BATCH_GRAPH;
+    if (flags & ~known) {
+        eng_set_error("%s: unsupported flags 0x%x (LEARNED_GRID: use libgrpp_batch_plan_set_predictions; "
+                      "GRADIENT: use libgrpp_batch_gradients_plan_create)", who, flags & ~known);
+        return 0;
+    }
+    return 1;
+}
+
 static eng_center_t *make_centers(int ncenters, const double *xyz, libgrpp_grpp_t *const *g, int *cidx, int *nidx)
 {
     eng_center_t *cs = (eng_center_t *) calloc(ncenters > 0 ? ncenters : 1, sizeof(eng_center_t));
@@ -330,6 +344,7 @@ int libgrpp_batch_plan_create(int nshells, libgrpp_shell_t *const *shells, int n
                               libgrpp_grpp_t *const *grpps, int flags, libgrpp_batch_plan_t **plan)
 {
     *plan = NULL;
+    if (!check_flags("batch_plan_create", flags)) return 2;
     if (!check_canonical(nshells, shells)) return 2;
     int *off = (int *) malloc(sizeof(int) * (nshells + 1));
     off[0] = 0;
@@ -357,8 +372,7 @@ int libgrpp_batch_plan_create(int nshells, libgrpp_shell_t *const *shells, int n
         }
     eng_plan_t *eng = NULL;
     long long n2 = (long long) dim * dim;
-    int eflags = ((flags & LIBGRPP_BATCH_LEARNED_GRID) ? ENG_F_LEARNED : 0) |
-                 ((flags & LIBGRPP_BATCH_RECORD_LEVELS) ? ENG_F_LEVELS : 0) |
+    int eflags = ((flags & LIBGRPP_BATCH_RECORD_LEVELS) ? ENG_F_LEVELS : 0) |
                  ((flags & LIBGRPP_BATCH_GRAPH) ? ENG_F_GRAPH : 0);
     int rc = eng_plan_create(nshells, shells, ncenters, cs, nidx, cidx, nit, it, n2, n2, ENG_ALL, eflags, &eng);
     free(off);
@@ -380,6 +394,7 @@ int libgrpp_batch_molecules_plan_create(int nmol, const int *shell_off, const in
                                         libgrpp_grpp_t *const *grpps, int flags, libgrpp_batch_plan_t **plan)
 {
     *plan = NULL;
+    if (!check_flags("batch_molecules_plan_create", flags)) return 2;
     if (nmol <= 0) {
         eng_set_error("batch_molecules: no molecules");
         return 2;
@@ -437,8 +452,7 @@ int libgrpp_batch_molecules_plan_create(int nmol, const int *shell_off, const in
         total += (long long) dim * dim;
     }
     eng_plan_t *eng = NULL;
-    int eflags = ((flags & LIBGRPP_BATCH_LEARNED_GRID) ? ENG_F_LEARNED : 0) |
-                 ((flags & LIBGRPP_BATCH_RECORD_LEVELS) ? ENG_F_LEVELS : 0) |
+    int eflags = ((flags & LIBGRPP_BATCH_RECORD_LEVELS) ? ENG_F_LEVELS : 0) |
                  ((flags & LIBGRPP_BATCH_GRAPH) ? ENG_F_GRAPH : 0);
     int rc = nit > 0x7fffffffLL ? 2
                                 : eng_plan_create(nshells, shells, ncenters, cs, nidx, cidx, (int) nit, it, total, total,
@@ -491,7 +505,7 @@ int libgrpp_batch_plan_dim(const libgrpp_batch_plan_t *p) { return p->dim; }
 int libgrpp_batch_plan_set_tuning(libgrpp_batch_plan_t *p, const libgrpp_batch_tuning_t *t)
 {
     eng_tuning_t e = {t->screen_probe, t->contract_threads, t->type1_threads, t->stream_layout, t->screen_split};
-    if (e.screen_split == 0) {   /* 0 keeps the plan's current split (a caller that predates the field) */
+    if (e.screen_split == 0) {   /* 0 keeps the plan's current split */
         eng_tuning_t cur;
         eng_plan_get_tuning(p->eng, &cur);
         e.screen_split = cur.screen_split;
@@ -616,6 +630,7 @@ int libgrpp_batch_gradients_plan_create(int nshells, libgrpp_shell_t *const *she
                                         libgrpp_grpp_t *const *grpps, int flags, libgrpp_batch_plan_t **plan)
 {
     *plan = NULL;
+    if (!check_flags("batch_gradients_plan_create", flags)) return 2;
     if (!check_canonical(nshells, shells)) return 2;
     grad_host_t *g = (grad_host_t *) calloc(1, sizeof(*g));
     const int ns1 = nshells + 1;

@@ -55,7 +55,7 @@ typedef struct {
 } eng_tuning_t;
 
 /* flags */
-enum { ENG_F_LEARNED = 1, ENG_F_STATS = 2, ENG_F_LEVELS = 4, ENG_F_GRAPH = 8 };
+enum { ENG_F_STATS = 2, ENG_F_LEVELS = 4, ENG_F_GRAPH = 8 };
 #define ENG_NFEAT 12   /* level-predictor features per radial row */
 
 int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int ncenter, const eng_center_t *centers,

@@ -5,8 +5,9 @@
 //
 //   k_tab_F     F^{C,A}_lambda(r_i): contracted radial functions of every shell A about every
 //               centre C on the level-ordered Log3 grid (radial_type2_integral.c:465-489);
-//   k_tab_wU    w_i U_p(r_i) for every semilocal / spin-orbit channel p (potential.c:70-81),
-//               kept as sign * sqrt|.| so that both operands of the product carry half of it;
+//   (host)      w_i U_p(r_i) for every semilocal / spin-orbit channel p (potential.c:70-81) is
+//               tabulated once per plan on the host with the C library's exp and pow, and kept as
+//               sign * sqrt|.| so that both operands of the product carry half of it;
 //   k_screen    the Shaw-Hill envelope estimate of every radial entry (screening.c:193-378), one
 //               wave per entry; it steers the quadrature exactly as in the reference;
 //   k_radial    the radial integrals of ALL shell pairs of a centre and channel as one FP64
@@ -38,6 +39,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -283,24 +285,6 @@ __global__ __launch_bounds__(256) void k_tab_F(const TabItem *__restrict__ items
 #pragma unroll
     for (int l = 0; l <= MAXLAM; l++)
         if (l <= it.lam) F[(size_t) (it.frow + l) * NPTS + i] = acc[l];
-}
-
-__global__ __launch_bounds__(256) void k_tab_wU(const int *__restrict__ pots, DevJob J, DevTables T,
-                                                double *__restrict__ wU)
-{
-    const int p = pots[blockIdx.y];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= NPTS) return;
-    const DPot P = J.pot[p];
-    const double r = T.r[i], r2 = r * r;
-    double v = 0.0;
-    for (int k = 0; k < P.nterm; k++)
-        v += J.tcoef[P.toff + k] * pow(r, J.tpow[P.toff + k] - 2) * exp(-J.tzeta[P.toff + k] * r2);
-    // stored as sign(u) sqrt|u|: the product stages r^n sqrt|u| on both operands and the sign on
-    // one, so that entry (a, b) rounds the same whichever side of a tile each row sits on, and the
-    // results do not depend on the order of the rows
-    const double u = T.w[i] * v;
-    wU[(size_t) blockIdx.y * NPTS + i] = copysign(sqrt(fabs(u)), u);
 }
 
 constexpr int RT = 64;   // radial tile edge (entries)
@@ -1718,6 +1702,44 @@ static int upload(const std::vector<T> &v, T **out, std::vector<void *> &keep)
     return 0;
 }
 
+// w_i U_p(r_i) of every channel p that has a radial table, on the host, as
+// copysign(sqrt|w U|, w U) (the product stages sqrt|w U| on both operands and the sign on one).
+// U is a sum of terms c r^(n-2) exp(-zeta r^2) that cancel, so one ulp of one term's exp moves
+// the result by one ulp of the largest term: on a g-g block of the reference's KCs case that alone
+// moved arep elements by 4.6e-9.  The terms are therefore evaluated with the C library's exp and
+// pow and accumulated as val = fma(c r^(n-2), exp(...), val), which is how the reference's
+// libgrpp_potential_value (potential.c:70-81) compiles with contraction, so that the table is the
+// reference's own potential on the same Log3 points.  Runs on up to 16 host threads.
+static void host_wU_table(const std::vector<int> &wpots, const std::vector<DPot> &pots, const std::vector<int> &tpow,
+                          const std::vector<double> &tcoef, const std::vector<double> &tzeta, std::vector<double> &out)
+{
+    const HostTables &H = host_tables();
+    out.assign(wpots.size() * (size_t) NPTS, 0.0);
+    auto work = [&](size_t b, size_t e) {
+        for (size_t q = b; q < e; q++) {
+            const DPot &p = pots[wpots[q]];
+            double *o = out.data() + q * NPTS;
+            for (int i = 0; i < NPTS; i++) {
+                const double r = H.r[i], r2 = r * r;
+                double v = 0.0;
+                for (int k = p.toff; k < p.toff + p.nterm; k++)
+                    v = std::fma(tcoef[k] * std::pow(r, tpow[k] - 2), std::exp(-tzeta[k] * r2), v);
+                const double u = H.w[i] * v;
+                o[i] = std::copysign(std::sqrt(std::fabs(u)), u);
+            }
+        }
+    };
+    const size_t n = wpots.size();
+    const size_t nt = std::min<size_t>(std::max(1u, std::min(16u, std::thread::hardware_concurrency())), std::max<size_t>(1, n / 8));
+    if (nt <= 1) {
+        work(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < nt; t++) th.emplace_back(work, n * t / nt, n * (t + 1) / nt);
+    for (auto &x : th) x.join();
+}
+
 typedef void (*contract_fn)(const PartItem *, const WorkItem *, DevJob, DevTables, double *, int, int, int, int);
 static contract_fn contract_kernel(int threads);
 
@@ -1830,7 +1852,6 @@ struct eng_plan {
            *d_delta = nullptr, *d_scr = nullptr, *d_Qc = nullptr;
     TabItem *d_tab = nullptr;
     int n_tab = 0;
-    int *d_wpots = nullptr;
     int n_wpots = 0;
     RowDesc *d_rows = nullptr;
     QJob *d_jobs = nullptr;
@@ -2360,7 +2381,6 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     rc |= dput(P, &d_slot_pb3, slot_pb3);
     rc |= dput(P, &d_ocv_tab, ocv_tab);
     rc |= dput(P, &P->d_tab, tab);
-    rc |= dput(P, &P->d_wpots, wpots);
     rc |= dput(P, &P->d_rows, rows);
     rc |= dput(P, &P->d_jobs, jobs);
     for (size_t i = 0; i < tiles.size(); i++) tiles[i].pad = (int) i;
@@ -2388,7 +2408,11 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     rc |= dalloc(P, &P->d_partial, (size_t) partial_len);
     rc |= dalloc(P, &P->d_partial_ct, (size_t) partial_len);
     rc |= dalloc(P, &P->d_F, (size_t) frows * NPTS);
-    rc |= dalloc(P, &P->d_wU, (size_t) wpots.size() * NPTS);
+    {
+        std::vector<double> wU;
+        host_wU_table(wpots, pots, tpow, tcoef, tzeta, wU);
+        rc |= dput(P, &P->d_wU, wU);
+    }
     rc |= dalloc(P, &P->d_Q, (size_t) qlen);
     rc |= dalloc(P, &P->d_scr, (size_t) qlen);
     rc |= dalloc(P, &P->d_Qc, (size_t) qlen);
@@ -2560,10 +2584,6 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
     if (P->n_tab > 0) {
         dim3 g(NPTS / 256, P->n_tab);
         hipLaunchKernelGGL(k_tab_F, g, dim3(256), 0, sd, P->d_tab, P->J, P->T, P->d_F);
-    }
-    if (P->n_wpots > 0) {
-        dim3 g(NPTS / 256, P->n_wpots);
-        hipLaunchKernelGGL(k_tab_wU, g, dim3(256), 0, sd, P->d_wpots, P->J, P->T, P->d_wU);
     }
     HIPCHK(hipEventRecord(P->ev[5], sd));
     if (P->n_proj > 0) hipLaunchKernelGGL(k_proj, dim3(P->n_proj), dim3(256), 0, sd, P->d_proj, P->J, P->T, P->d_P);

@@ -23,6 +23,7 @@ ORC_SO = os.path.join(HERE, "liborc.so")
 ORC_NOFMA_SO = os.path.join(HERE, "liborc_nofma.so")
 REF_SO = os.path.join(HERE, "_ref", "libgrpp_ref.so")
 REF_NOFMA_SO = os.path.join(HERE, "_ref", "libgrpp_ref_nofma.so")
+REF_ASSOC_SO = os.path.join(HERE, "_ref", "libgrpp_ref_assoc.so")
 
 ORC_NAMES = dict(new_shell="orc_new_shell", delete_shell="orc_delete_shell", new_potential="orc_new_potential",
                  delete_potential="orc_delete_potential", new_grpp="orc_new_grpp", set_local="orc_grpp_set_local",
@@ -31,7 +32,8 @@ ORC_NAMES = dict(new_shell="orc_new_shell", delete_shell="orc_delete_shell", new
 
 
 class RadialOpts(C.Structure):
-    _fields_ = [("tol", C.c_double), ("screening", C.c_int), ("min_level", C.c_int), ("fixed_level", C.c_int)]
+    _fields_ = [("tol", C.c_double), ("screening", C.c_int), ("min_level", C.c_int), ("fixed_level", C.c_int),
+                ("engine_order", C.c_int)]
 
 
 def build(ref: bool = True) -> None:
@@ -142,6 +144,16 @@ def reference_nofma() -> Optional[CpuLib]:
     if "ref_nofma" not in _cache:
         _cache["ref_nofma"] = CpuLib(REF_NOFMA_SO, "reference") if os.path.exists(REF_NOFMA_SO) else None
     return _cache["ref_nofma"]
+
+
+def reference_assoc() -> Optional[CpuLib]:
+    """The genuine reference sources compiled with reassociation allowed (gcc then vectorises the
+    quadrature point sums into four-lane partial sums): the reference's own rounding in another
+    summation order.  Together with reference_nofma() it bounds how far the reference's arithmetic
+    fixes an ill-conditioned input."""
+    if "ref_assoc" not in _cache:
+        _cache["ref_assoc"] = CpuLib(REF_ASSOC_SO, "reference") if os.path.exists(REF_ASSOC_SO) else None
+    return _cache["ref_assoc"]
 
 
 # ---------------------------------------------------------------- per-call entry points

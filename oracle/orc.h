@@ -62,6 +62,9 @@ typedef struct {
     int screening;       /* 1: Shaw-Hill screening as in screening.c:193-248; 0: none          */
     int min_level;       /* never stop before this doubling level (0 = reference behaviour)   */
     int fixed_level;     /* >=0: evaluate exactly this level, no adaptivity (diagnostics)      */
+    int engine_order;    /* diagnostics: bit 0 each level's new points in a fresh sum added to
+                            S/2, bit 1 the point products as F1 (r^nA s) * F2 (r^nB sign s) with
+                            s = sqrt|w U|, bit 2 new points summed in groups of four (0 = reference) */
 } orc_radial_opts_t;
 
 extern orc_radial_opts_t orc_radial_opts;

@@ -15,7 +15,7 @@
 #define NLEVEL 9            /* 31, 63, ..., 8191 points */
 #define GRID_TOTAL 8191
 
-orc_radial_opts_t orc_radial_opts = {1e-16, 1, 0, -1};
+orc_radial_opts_t orc_radial_opts = {1e-16, 1, 0, -1, 0};
 
 typedef struct {
     int nfill;              /* points tabulated so far (level-ordered) */
@@ -153,6 +153,34 @@ static inline double point_sum(const double *w, const double *U, const double *F
     return s;
 }
 
+/* the same sum in another arithmetic order (orc_radial_opts.engine_order, diagnostics only): the
+ * level's points in a fresh sum (bit 0) and/or in groups of four (bit 2), each point's product as
+ * (F1 r^nA s)(F2 r^nB sign s), s = sqrt|w U|, N = nA + nB with nA the smallest allowed (bit 1) */
+static double point_sum_alt(const grid_t *g, const double *F1, const double *F2, int N, int b, int e, double s,
+                            int mode)
+{
+    int nb = N - g->ket->L < 0 ? 0 : N - g->ket->L, na = N - nb;
+    if (na > g->bra->L) { na = g->bra->L; nb = N - na; }
+    double acc = (mode & 1) ? 0.0 : s, four = 0.0;
+    int cnt = 0;
+    for (int i = b; i < e; i++) {
+        double p;
+        if (mode & 2) {
+            double wu = g->w[i] * g->U[i], sq = sqrt(fabs(wu)), sg = wu < 0 ? -sq : sq;
+            p = (F1[i] * (pow(g->r[i], na) * sq)) * (F2[i] * (pow(g->r[i], nb) * sg));
+        } else {
+            p = g->w[i] * g->U[i] * F1[i] * F2[i] * g->rN[N * GRID_TOTAL + i];
+        }
+        if (mode & 4) {
+            four += p;
+            if (++cnt == 4 || i == e - 1) { acc += four; four = 0.0; cnt = 0; }
+        } else {
+            acc += p;
+        }
+    }
+    return (mode & 1) ? s + acc : acc;
+}
+
 /* adaptive nested-doubling quadrature of one entry (radial_type2_integral.c:242-318) */
 static double integrate_entry(grid_t *g, int n, int l1, int l2, int *level_out)
 {
@@ -172,7 +200,7 @@ static double integrate_entry(grid_t *g, int n, int l1, int l2, int *level_out)
     const double *F1 = g->F1 + l1 * GRID_TOTAL, *F2 = g->F2 + l2 * GRID_TOTAL, *rN = g->rN + n * GRID_TOTAL;
     int nr = MIN_GRID, level = 0;
     double sum = 0.0, prev;
-    sum = point_sum(w, U, F1, F2, rN, 0, nr, sum);
+    sum = o->engine_order ? point_sum_alt(g, F1, F2, n, 0, nr, sum, o->engine_order) : point_sum(w, U, F1, F2, rN, 0, nr, sum);
     if (o->fixed_level == 0) { if (level_out) *level_out = 0; return sum; }
     int converged = 0;
     do {
@@ -182,7 +210,8 @@ static double integrate_entry(grid_t *g, int n, int l1, int l2, int *level_out)
         prev = sum;
         sum = 0.5 * sum;
         grid_expand(g, nr);
-        sum = point_sum(w, U, F1, F2, rN, idx, nr, sum);
+        sum = o->engine_order ? point_sum_alt(g, F1, F2, n, idx, nr, sum, o->engine_order)
+                              : point_sum(w, U, F1, F2, rN, idx, nr, sum);
         level++;
         if (o->fixed_level >= 0) {
             if (level >= o->fixed_level) break;

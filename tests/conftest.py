@@ -47,21 +47,54 @@ def kcs():
     return io.load_system(os.path.join(GOLDEN, "KCs"))
 
 
-@pytest.fixture(scope="session")
-def kcs_oracle(kcs):
-    """oracle matrices of KCs with and without FMA contraction (the reference formulation's
-    rounding spread), computed once per session"""
-    import oracle
-    return oracle.oracle().matrices(kcs), oracle.oracle_nofma().matrices(kcs)
+# Where the reference's own arithmetic fixes an ill-conditioned input only loosely, its results
+# move when its sources are compiled with contraction off or with reassociation allowed (vectorised
+# quadrature point sums).  The largest such move over a shell-pair block is that block's noise
+# floor (tests/golden/make_golden.py); an engine element then passes the parity rule or lies within
+# FLOOR_FACTOR floors of the reference.  The engine's summation order (matrix-core products) is a
+# third order: measured on every fixture it stays within 2.5 floors.
+FLOOR_FACTOR = 3.0
 
 
-def within_noise(got, want, spread, factor=10.0):
-    """element-wise: the parity rule, or |d| <= 1e-10 + factor * (rounding spread of the reference
-    formulation at that element)"""
+def parity_rule(got, want):
+    """|d| <= 1e-10, or |d| <= 1e-9 |want| where |want| > 1"""
     import numpy as np
     d = np.abs(got - want)
-    rule = (d <= 1e-10) | ((np.abs(want) > 1.0) & (d <= 1e-9 * np.abs(want)))
-    return rule | (d <= 1e-10 + factor * spread)
+    return (d <= 1e-10) | ((np.abs(want) > 1.0) & (d <= 1e-9 * np.abs(want)))
+
+
+def within_floor(got, want, floor, factor=FLOOR_FACTOR):
+    """element-wise: the parity rule, or |d| <= 1e-10 + factor * floor (floor broadcastable)"""
+    import numpy as np
+    return parity_rule(got, want) | (np.abs(got - want) <= 1e-10 + factor * floor)
+
+
+def block_floor_matrix(sysm, floor):
+    """dense dim x dim floor from a per-shell-pair floor[ns, ns] (lower triangle i >= j filled; the
+    upper triangle mirrors it)"""
+    import numpy as np
+    off = sysm.offsets() + [sysm.dim]
+    ns = len(sysm.shells)
+    F = np.zeros((sysm.dim, sysm.dim))
+    for i in range(ns):
+        for j in range(i + 1):
+            v = max(float(floor[i, j]), float(floor[j, i]))
+            F[off[i]:off[i + 1], off[j]:off[j + 1]] = v
+            F[off[j]:off[j + 1], off[i]:off[i + 1]] = v
+    return F
+
+
+def error_summary(got, want, floor):
+    """max abs error, max relative error over |want| > 1 and the largest error / floor ratio among
+    elements outside the parity rule (0 when every element meets the rule)"""
+    import numpy as np
+    d = np.abs(got - want)
+    big = np.abs(want) > 1.0
+    rel = float((d[big] / np.abs(want[big])).max(initial=0.0))
+    out = ~parity_rule(got, want)
+    fl = np.broadcast_to(floor, d.shape)
+    ratio = float(((d[out] - 1e-10) / np.maximum(fl[out], 1e-300)).max(initial=0.0))
+    return float(d.max(initial=0.0)), rel, ratio
 
 
 def golden_matrix(case, name):

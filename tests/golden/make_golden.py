@@ -14,8 +14,14 @@ the reference sources; the fixtures are data only).
 
 * ``heavy_cluster_reference.npz`` / ``ragged_reference.npz`` -- full-GRPP blocks of sampled
   ordered shell pairs (all centres) of the heavy_cluster (seed 0) and 200-molecule ragged (seed 0)
-  workloads, from the genuine reference and from the same sources compiled with floating-point
-  contraction off (the per-element spread of the two is the reference's own rounding floor).
+  workloads from the genuine reference, with each block's noise floor: how far the same sources
+  compiled with contraction off, or with reassociation allowed (vectorised point sums), move it;
+* ``compact_reference.npz`` -- the same for every block of two heavy_cluster atoms in a 4-bohr box;
+* ``KCs/block_floor.npz``, ``KCs/gradient_block_floor.npz`` -- per-block floors of the KCs goldens;
+* ``high_l_percall_reference.npz`` -- per-call blocks of shells up to l = 8 (high_l_calls);
+* ``per_term_gradient_reference.npz`` -- the per-term gradient entry points on H2O tasks.
+* ``ThO/``, ``UO2/`` -- the reference's ThO and UO2 test cases (inputs, gzipped goldens) and the
+  per-block floors of the reference's own arithmetic on them (``heavy_goldens``).
 
     python tests/golden/make_golden.py [fixture ...]
 """
@@ -129,32 +135,49 @@ def h2o_pergrad_reference():
                         values=np.concatenate(out), lengths=np.array([len(o) for o in out]))
 
 
-def _sampled_blocks(systems, picks):
-    """Full-GRPP blocks of the genuine reference (FMA build) and of the same sources compiled with
-    contraction off, for sampled ordered shell pairs: picks = [(molecule, [pair index i*ns + j])].
-    Returns flat arrays: molecule, pair, offset of each block in the flattened 4-operator values."""
+def _reference_builds():
     import oracle
-    ref, nof = oracle.reference(), oracle.reference_nofma()
-    if ref is None or nof is None:
+    libs = oracle.reference(), oracle.reference_nofma(), oracle.reference_assoc()
+    if any(x is None for x in libs):
         raise SystemExit("oracle/_ref not built: run make -C oracle ref")
-    mol, pair, offs, vals, valn = [], [], [0], [], []
+    return libs
+
+
+def _sampled_blocks(systems, picks, nthreads=0):
+    """Full-GRPP blocks of the genuine reference (FMA build) for sampled ordered shell pairs, picks =
+    [(molecule, [pair index i*ns + j])], with each block's noise floor: the largest deviation of the
+    same sources compiled with contraction off, or with reassociation allowed, from the reference on
+    that block and operator.  Returns flat arrays: molecule, pair, offset of each block in the
+    flattened 4-operator values, the values and floor[block, operator]."""
+    builds = _reference_builds()
+    mol, pair, offs, vals, floor = [], [], [0], [], []
     for m, pairs in picks:
         sysm = systems[m]
         pairs = np.sort(np.asarray(pairs, dtype=np.int32))
-        R = ref.matrices(sysm, nthreads=os.cpu_count(), pairs=pairs)
-        N = nof.matrices(sysm, nthreads=os.cpu_count(), pairs=pairs)
+        R, N, A = [b.matrices(sysm, nthreads=nthreads or os.cpu_count(), pairs=pairs) for b in builds]
         off = sysm.offsets() + [sysm.dim]
         ns = len(sysm.shells)
         for p in pairs:
             i, j = divmod(int(p), ns)
-            blk = lambda X: np.array([X[k][off[i]:off[i + 1], off[j]:off[j + 1]] for k in range(4)]).ravel()
+            sl = (slice(off[i], off[i + 1]), slice(off[j], off[j + 1]))
             mol.append(m)
             pair.append(int(p))
-            vals.append(blk(R))
-            valn.append(blk(N))
+            vals.append(np.array([R[k][sl] for k in range(4)]).ravel())
+            floor.append([max(np.abs(R[k][sl] - N[k][sl]).max(), np.abs(R[k][sl] - A[k][sl]).max()) for k in range(4)])
             offs.append(offs[-1] + vals[-1].size)
     return dict(molecule=np.array(mol, np.int32), pair=np.array(pair, np.int32), offset=np.array(offs, np.int64),
-                ref=np.concatenate(vals), ref_nofma=np.concatenate(valn))
+                ref=np.concatenate(vals), floor=np.array(floor))
+
+
+def compact_reference():
+    """heavy_cluster class squeezed to two atoms 4.0-bohr box (configs.heavy_cluster(natom=2,
+    box=4.0)): the whole matrices of the genuine reference and the noise floor of every ordered
+    shell pair block (as in _sampled_blocks), the most ill-conditioned case of the tests"""
+    from libgrpp_amd import configs
+    sysm = configs.heavy_cluster(natom=2, box=4.0)
+    ns = len(sysm.shells)
+    d = _sampled_blocks([sysm], [(0, np.arange(ns * ns))])
+    np.savez_compressed(os.path.join(HERE, "compact_reference.npz"), **d)
 
 
 def heavy_cluster_reference(npair: int = 128):
@@ -168,23 +191,225 @@ def heavy_cluster_reference(npair: int = 128):
     np.savez_compressed(os.path.join(HERE, "heavy_cluster_reference.npz"), **d)
 
 
-def ragged_reference(nmol: int = 8, npair: int = 12):
-    """the 200-molecule ragged workload (seed 0): nmol molecules picked at random, npair random
-    ordered shell pairs of each with all of the molecule's centres"""
+def _ragged_job(job):
+    """one (molecule, ordered pair) of the ragged workload through both reference builds, one thread"""
+    m, p = job
+    from libgrpp_amd import configs
+    d = _sampled_blocks(configs.ragged_molecules(), [(m, [p])], nthreads=1)
+    return m, p, d
+
+
+def ragged_reference(nmol: int = 12, npair: int = 12, lsum_max: int = 6):
+    """the 200-molecule ragged workload (seed 0): nmol molecules picked at random and, in each,
+    npair random ordered shell pairs with l_A + l_B <= lsum_max, each with all of the molecule's
+    centres.  The bound keeps the reference's own cost in minutes: it spends 2.3 s on one d-d pair
+    of a six-centre molecule of this set and 211 s on one g-g pair."""
+    from multiprocessing import Pool
     from libgrpp_amd import configs
     mols = configs.ragged_molecules()
     rng = np.random.default_rng(2027)
     pick = sorted(int(x) for x in rng.choice(len(mols), nmol, replace=False))
-    picks = []
+    jobs = []
     for m in pick:
         ns = len(mols[m].shells)
-        picks.append((m, rng.permutation(ns * ns)[:npair]))
-    d = _sampled_blocks(mols, picks)
-    np.savez_compressed(os.path.join(HERE, "ragged_reference.npz"), **d)
+        ok = [p for p in rng.permutation(ns * ns)
+              if mols[m].shells[p // ns].L + mols[m].shells[p % ns].L <= lsum_max]
+        jobs += [(m, int(p)) for p in sorted(ok[:npair])]
+    with Pool(os.cpu_count()) as pool:
+        res = pool.map(_ragged_job, jobs, chunksize=1)
+    mol, pair, offs, vals, floor = [], [], [0], [], []
+    for m, p, d in res:
+        mol.append(m)
+        pair.append(p)
+        vals.append(d["ref"])
+        floor.append(d["floor"][0])
+        offs.append(offs[-1] + d["ref"].size)
+    np.savez_compressed(os.path.join(HERE, "ragged_reference.npz"), molecule=np.array(mol, np.int32),
+                        pair=np.array(pair, np.int32), offset=np.array(offs, np.int64), ref=np.concatenate(vals),
+                        floor=np.array(floor))
+
+
+def heavy_goldens(cases=("ThO", "UO2")):
+    """The reference's ThO and UO2 test cases (test/<case>/: inputs and the four full-GRPP golden
+    matrices, gzipped) plus <case>/block_floor.npz: per ordered shell pair and operator, the largest
+    of |reference - contraction-off reference|, |reference - reassociated reference| and
+    |reference - golden| over the pair's block of the lower triangle.  That is how far the reference's own arithmetic fixes each block: the compiled
+    reference itself misses these goldens by up to 4.3e-7 (ThO) and 2.0e-5 (UO2) on arep."""
+    import gzip
+    import shutil
+    from libgrpp_amd import io
+    ref, nof, aso = _reference_builds()
+    for case in cases:
+        src = os.path.join(os.path.dirname(REF), "test", case)
+        dst = os.path.join(HERE, case)
+        os.makedirs(dst, exist_ok=True)
+        for f in ("basis.inp", "ecp.inp", "molecule.xyz"):
+            shutil.copyfile(os.path.join(src, f), os.path.join(dst, f))
+        for nm in OPS:
+            with open(os.path.join(src, nm + ".txt"), "rb") as fi, open(os.path.join(dst, nm + ".txt.gz"), "wb") as raw, \
+                    gzip.GzipFile(fileobj=raw, mode="wb", compresslevel=9, mtime=0) as fo:
+                shutil.copyfileobj(fi, fo)
+        sysm = io.load_system(dst)
+        R = ref.matrices(sysm, nthreads=os.cpu_count())
+        N = nof.matrices(sysm, nthreads=os.cpu_count())
+        A = aso.matrices(sysm, nthreads=os.cpu_count())
+        off = sysm.offsets() + [sysm.dim]
+        ns = len(sysm.shells)
+        floor = np.zeros((4, ns, ns))
+        lower = np.tril(np.ones((sysm.dim, sysm.dim), bool))
+        for k, nm in enumerate(OPS):
+            G = io.read_lower_triangle(os.path.join(src, nm + ".txt"))
+            D = np.where(lower, np.maximum(np.maximum(np.abs(R[k] - N[k]), np.abs(R[k] - A[k])), np.abs(R[k] - G)), 0.0)
+            for i in range(ns):
+                for j in range(i + 1):
+                    floor[k, i, j] = D[off[i]:off[i + 1], off[j]:off[j + 1]].max()
+        np.savez_compressed(os.path.join(dst, "block_floor.npz"), floor=floor.astype(np.float32))
+
+
+# (kind, shell a, shell b, centre c, point d, channel): H2O shells O 0..13, H1 14..17, H2 18..21;
+# channel = index into the centre's arep / esop list (type-2 / spin-orbit), unused otherwise
+PER_TERM_TASKS = [("type1", 0, 14, 2, 1, 0), ("type1", 5, 6, 1, 0, 0), ("type1", 15, 3, 0, 1, 0),
+                  ("type1", 9, 19, 0, 0, 0), ("type2", 0, 14, 2, 2, 0), ("type2", 2, 7, 1, 1, 0),
+                  ("type2", 16, 4, 0, 1, 0), ("type2", 10, 21, 0, 2, 0), ("so", 0, 14, 2, 1, 0),
+                  ("so", 8, 15, 0, 1, 0), ("so", 17, 12, 0, 1, 0), ("so", 13, 20, 0, 0, 0),
+                  ("oc", 1, 15, 2, 1, 0), ("oc", 6, 9, 1, 1, 0), ("oc", 18, 2, 0, 2, 0), ("oc", 11, 16, 0, 0, 0)]
+
+
+def per_term_gradient(lib, fac, kind, A, B, Cpos, arg, D):
+    """libgrpp_{type1,type2,spin_orbit,outercore_potential}_integrals_gradient of ``lib`` (the
+    genuine reference or this project's library: same names and signatures, libgrpp.h:178-231) for
+    shells A, B, GRPP centre Cpos and the point D differentiated; arg = Potential (type1 / type2 /
+    so) or [(Potential, Shell)] (oc).  Returns [op][xyz] blocks: 1 operator for type1 / type2, 3
+    for so, 4 (arep, so_x, so_y, so_z) for oc."""
+    from libgrpp_amd.abi import PPOT, PSHELL, as_dptr, darr, dptr
+    pp = C.POINTER(dptr)
+    na, nb = A.cart_size, B.cart_size
+    nop = {"type1": 1, "type2": 1, "so": 3, "oc": 4}[kind]
+    arrs = [[np.zeros(na * nb) for _ in range(3)] for _ in range(nop)]
+    ptrs = [(dptr * 3)(*[as_dptr(x) for x in ar]) for ar in arrs]
+    a, b = fac.shell(A), fac.shell(B)
+    if kind in ("type1", "type2"):
+        fn = getattr(lib, f"libgrpp_{kind}_integrals_gradient")
+        fn.restype, fn.argtypes = None, [PSHELL, PSHELL, dptr, PPOT, dptr, pp]
+        pot = fac.potential(arg)
+        fn(a, b, darr(Cpos), pot, darr(D), ptrs[0])
+        fac.f["delete_potential"](pot)
+    elif kind == "so":
+        fn = lib.libgrpp_spin_orbit_integrals_gradient
+        fn.restype, fn.argtypes = None, [PSHELL, PSHELL, dptr, PPOT, dptr, pp, pp, pp]
+        pot = fac.potential(arg)
+        fn(a, b, darr(Cpos), pot, darr(D), *ptrs)
+        fac.f["delete_potential"](pot)
+    else:
+        fn = lib.libgrpp_outercore_potential_integrals_gradient
+        fn.restype = None
+        fn.argtypes = [PSHELL, PSHELL, dptr, C.c_int, C.POINTER(PPOT), C.POINTER(PSHELL), dptr, pp, pp, pp, pp]
+        n = len(arg)
+        pots = (PPOT * max(1, n))(*[fac.potential(pt) for pt, _ in arg])
+        shs = (PSHELL * max(1, n))(*[fac.shell(sh) for _, sh in arg])
+        fn(a, b, darr(Cpos), n, pots, shs, darr(D), *ptrs)
+        for i in range(n):
+            fac.f["delete_potential"](pots[i])
+            fac.f["delete_shell"](shs[i])
+    fac.f["delete_shell"](a)
+    fac.f["delete_shell"](b)
+    return np.array([[x.reshape(na, nb) for x in ar] for ar in arrs])
+
+
+def per_term_operand(sysm, kind, c, ch):
+    g = sysm.centers[c][1]
+    return {"type1": lambda: g.local, "type2": lambda: g.arep[ch], "so": lambda: g.esop[ch],
+            "oc": lambda: g.oc}[kind]()
+
+
+def per_term_gradients():
+    """per_term_gradient_reference.npz: the four per-term gradient entry points of the genuine
+    reference (grpp_integrals_gradient.c:123-272) on PER_TERM_TASKS of the H2O case, covering the
+    A/B/C coincidences of the point differentiated and, for spin-orbit, the wrapper's own rescaling
+    of the potential by (2L+1)/2 (grpp_integrals_gradient.c:226-237)"""
+    import oracle
+    from libgrpp_amd import io
+    ref = oracle.reference()
+    sysm = io.load_system(os.path.join(HERE, "H2O"))
+    atoms = [c for c, _ in sysm.centers]
+    vals, lens = [], []
+    for kind, a, b, c, d, ch in PER_TERM_TASKS:
+        v = per_term_gradient(ref.lib, ref.fac, kind, sysm.shells[a], sysm.shells[b], atoms[c],
+                              per_term_operand(sysm, kind, c, ch), atoms[d]).ravel()
+        vals.append(v)
+        lens.append(v.size)
+    np.savez_compressed(os.path.join(HERE, "per_term_gradient_reference.npz"), values=np.concatenate(vals),
+                        lengths=np.array(lens))
+
+
+def kcs_gradient_floors():
+    """KCs/gradient_block_floor.npz: for the two KCs gradient goldens kept in KCs/ (d/dx and d/dy of
+    the Cs atom: arep_grad_1x, so_z_grad_1y), per lower-triangle shell-pair block the largest of
+    |reference - contraction-off reference|, |reference - reassociated reference| and
+    |reference - golden| (gradients by the reference's libgrpp_full_grpp_integrals_gradient)"""
+    from libgrpp_amd import io
+    builds = _reference_builds()
+    sysm = io.load_system(os.path.join(HERE, "KCs"))
+    R, N, A = [b.gradients(sysm, nthreads=os.cpu_count())[0] for b in builds]
+    off = sysm.offsets() + [sysm.dim]
+    ns = len(sysm.shells)
+    out = {}
+    for name, (op, at, c) in {"arep_grad_1x": (0, 1, 0), "so_z_grad_1y": (3, 1, 1)}.items():
+        k = 3 * at + c
+        G = io.read_lower_triangle(os.path.join(HERE, "KCs", name + ".txt.gz"))
+        D = np.tril(np.maximum(np.maximum(np.abs(R[op, k] - N[op, k]), np.abs(R[op, k] - A[op, k])),
+                               np.abs(np.tril(R[op, k]) - G)))
+        fl = np.zeros((ns, ns))
+        for i in range(ns):
+            for j in range(i + 1):
+                fl[i, j] = D[off[i]:off[i + 1], off[j]:off[j + 1]].max()
+        out[name] = fl.astype(np.float32)
+    np.savez_compressed(os.path.join(HERE, "KCs", "gradient_block_floor.npz"), **out)
+
+
+def high_l_percall():
+    """high_l_percall_reference.npz: per-call type-1 / type-2 / spin-orbit blocks of shells up to
+    l = 8 and channels up to L = 5 near the GRPP centre (the inputs of
+    tests/test_gpu_parity.py::test_per_call_high_angular_momentum, rebuilt here from the same seed),
+    from the genuine reference, with each call's noise floor from the contraction-off and
+    reassociated builds"""
+    from libgrpp_amd import configs
+    ref, nof, aso = _reference_builds()
+    out = {}
+    for name, A, B, C, what, arg in high_l_calls():
+        R = np.asarray(ref.per_call(what, A, B, C, arg))
+        fl = max(np.abs(R - np.asarray(nof.per_call(what, A, B, C, arg))).max(),
+                 np.abs(R - np.asarray(aso.per_call(what, A, B, C, arg))).max())
+        out[name] = R
+        out[name + "_floor"] = np.array(fl)
+    np.savez_compressed(os.path.join(HERE, "high_l_percall_reference.npz"), **out)
+
+
+def high_l_calls():
+    """(name, A, B, C, kind, operand) of the high-l per-call cases, seed 2"""
+    from libgrpp_amd import configs, io
+    rng = np.random.default_rng(2)
+    C = (0.0, 0.0, 0.0)
+    g = io.GRPP(local=configs._channel(rng, 6, 0, 3), arep=[configs._channel(rng, L, 0, 3) for L in range(6)],
+                esop=[configs._channel(rng, L, 0, 3) for L in range(1, 6)])
+    cases = [(8, 0, [("type1", g.local), ("type2", g.arep[5]), ("type2", g.arep[2]), ("so", g.esop[4])]),
+             (7, 3, [("type1", g.local), ("type2", g.arep[3]), ("so", g.esop[1])])]
+    out = []
+    for la, lb, calls in cases:
+        A = configs._shell(rng, la, (0.6, -0.4, 0.5), 2, (0.5, 20.0))
+        B = configs._shell(rng, lb, (-0.3, 0.8, 1.1), 2, (0.5, 20.0))
+        for what, arg in calls:
+            out.append((f"{what}_L{arg.L}_{la}{lb}", A, B, C, what, arg))
+    return out
+
+
+OPS = ["arep", "so_x", "so_y", "so_z"]
 
 
 FIXTURES = {"tables": tables_sample, "small_pair": small_pair_reference, "h2o_pergrad": h2o_pergrad_reference,
-            "heavy_cluster": heavy_cluster_reference, "ragged": ragged_reference}
+            "heavy_cluster": heavy_cluster_reference, "ragged": ragged_reference, "heavy_goldens": heavy_goldens,
+            "compact": compact_reference, "kcs_floors": lambda: (heavy_goldens(("KCs",)), kcs_gradient_floors()),
+            "high_l": high_l_percall, "per_term_gradients": per_term_gradients}
 
 
 if __name__ == "__main__":

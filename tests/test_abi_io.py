@@ -96,3 +96,29 @@ def test_contracted_normalisation():
         c, a = [0.3, 0.9, 0.4], [5.0, 0.9, 0.11]
         f = io.contracted_norm(L, c, a)
         assert io.contracted_norm(L, [x * f for x in c], a) == pytest.approx(1.0, rel=1e-13)
+
+
+def test_batch_plans_refuse_reserved_flags():
+    """LIBGRPP_BATCH_LEARNED_GRID and LIBGRPP_BATCH_GRADIENT are refused with an error before any
+    device work (host code: no GPU needed), for all three plan constructors"""
+    if not os.path.exists(LIB):
+        pytest.skip("libgrpp_amd.so not built")
+    from libgrpp_amd.abi import LIBGRPP_NAMES, PGRPP, PSHELL, ObjectFactory
+    lib = C.CDLL(LIB)
+    lib.libgrpp_last_error.restype = C.c_char_p
+    f = ObjectFactory(lib, LIBGRPP_NAMES)
+    sh = (PSHELL * 1)(f.shell(io.Shell(0, (0.0, 0.0, 0.0), [1.0], [1.0])))
+    g = (PGRPP * 1)(f.grpp(io.GRPP(local=io.Potential(0, 0, [2], [1.0], [1.0]))))
+    xyz = (C.c_double * 3)(0.0, 0.0, 0.0)
+    plan = C.c_void_p()
+    for flags in (1, 2, 3, 1 | 16):
+        rc = lib.libgrpp_batch_plan_create(1, sh, 1, xyz, g, flags, C.byref(plan))
+        assert rc != 0 and not plan.value
+        assert b"unsupported flags" in lib.libgrpp_last_error()
+        rc = lib.libgrpp_batch_gradients_plan_create(1, sh, 1, xyz, g, flags, C.byref(plan))
+        assert rc != 0 and not plan.value
+        so, co = (C.c_int * 2)(0, 1), (C.c_int * 2)(0, 1)
+        rc = lib.libgrpp_batch_molecules_plan_create(1, so, co, sh, xyz, g, flags, C.byref(plan))
+        assert rc != 0 and not plan.value
+    f.free_grpp(g[0])
+    f.f["delete_shell"](sh[0])
