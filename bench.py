@@ -1,14 +1,19 @@
 #!/usr/bin/env python3
 """Throughput of the batched FP64 GRPP matrix build on one MI355X.
 
-One step = one full-GRPP matrix build (type-1 + semilocal type-2 + spin-orbit + outercore,
-all ordered shell pairs x all GRPP centres) of the ``heavy_cluster`` workload: 8 heavy atoms in a
-12-bohr cube, GRPP with semilocal L=0..4 + local, SO L=1..4, outercore s/p projectors, basis
-4s4p3d2f1g with 6 primitives per shell (112 shells, dim 552, 100,352 shell-pair x centre tasks),
-synthetic, seed 0.  Inputs are packed and resident in HBM before timing; a step runs the kernels.
-The build is timed twice: with the plain adaptive method, then with radial tables ordered by the
-learned level predictor (trained beforehand, untimed, on seeds 1-4); ``value`` is the latter and
-``learned_grid`` holds the comparison.
+One step = one full-GRPP matrix build (type-1 + semilocal type-2 + spin-orbit + outercore, all
+ordered shell pairs x all GRPP centres) of the ``ragged_molecules`` workload: 200 independent
+molecules of 2-30 atoms in one plan, 1-6 GRPP centres each (semilocal L <= 5 + local, SO, outercore
+s/p), Cartesian shells l = 0..6 drawn with probability proportional to 1/(l+1) and 1-8 primitives
+(11,051 shells, 666 centres, 2.70e6 shell-pair x centre tasks), synthetic, seed 0.
+``--workload heavy_cluster`` times the 8-atom cluster instead (100,352 tasks).  Inputs are packed and
+resident in HBM before timing; a step runs the kernels.  The build is timed twice: with the plain
+adaptive method, then with radial tables ordered by the learned level predictor (trained
+beforehand, untimed, on seeds 1-4); ``value`` is the latter and ``learned_grid`` holds the
+comparison.  ``parity`` compares the matrices with the genuine reference's blocks for the sampled
+ordered pairs of tests/golden (per operator: max abs / relative error and the largest error over
+the reference's own noise floor of the block).  ``percall`` times the reference's own usage
+pattern through this library (host threads calling libgrpp_full_grpp_integrals per task).
 
 Prints one JSON line (see the keys below).  --gpus N with N > 1 is not supported: the build is
 single-GPU (tasks would shard by shell-pair rows without communication).
@@ -56,47 +61,210 @@ def traffic_bytes(kernel: str, workload: str):
     return None
 
 
-def cpu_baseline(sysm, budget_s: float = 15.0):
+def fp64_counters(workload: str, step_s: float):
+    """Issued FP64 work per build from the committed rocprofv3 counter summary of ``workload``
+    (profiles/counters_<workload>.txt: wave instructions per launch): FLOP = 64 lanes x (2 FMA + MUL
+    + ADD + TRANS) + 512 x MFMA_MOPS_F64, an upper bound (every lane counted active).  Returns the
+    per-kernel figures and the whole build's fraction of the FP64 peak at the measured step time;
+    None when no summary of that workload is present."""
+    path = os.path.join(ROOT, "profiles", f"counters_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    rows, cols = {}, None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 2 and f[0] == "kernel":
+            cols = f[1:]
+        elif cols and len(f) == len(cols) + 1 and f[0].startswith("k_"):
+            try:
+                rows.setdefault(f[0], {}).update({c: float(v) for c, v in zip(cols, f[1:])})
+            except ValueError:
+                pass
+    per = {}
+    for k, r in rows.items():
+        if "VALU_FMA_F64" not in r:
+            continue
+        fl = 64.0 * (2 * r["VALU_FMA_F64"] + r["VALU_MUL_F64"] + r["VALU_ADD_F64"] + r["VALU_TRANS_F64"]) + \
+            512.0 * r.get("VALU_MFMA_MOPS_F64", 0.0)
+        per[k] = fl
+    if not per:
+        return None
+    total = sum(per.values())
+    return {"source": os.path.relpath(path, ROOT), "issued_flops_per_build": total,
+            "per_kernel": per, "build_achieved_tflops": total / step_s / 1e12,
+            "build_frac": total / step_s / 1e12 / FP64_PEAK_TFLOPS,
+            "note": "upper bound: 64 active lanes per wave instruction"}
+
+
+def host_info():
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def _threads():
+    """host threads for the CPU legs: the process's CPU affinity, at most 16 (one GPU's share)"""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return max(1, min(16, n))
+
+
+LSUM_MAX = 5   # CPU-side samples: ordered pairs with l_A + l_B <= 5 only
+
+
+def cpu_sample(sysm, rng_seed=1):
+    """the fixed sample both CPU-side legs use: (molecule index, sorted ordered-pair indices) in a
+    fixed random order; for one system, molecule index 0.  Only pairs with l_A + l_B <= LSUM_MAX
+    are drawn: the reference spends minutes on one g-g pair of a six-centre molecule, which would
+    make a bounded CPU leg unbounded.  The restriction favours the CPU (cheap pairs)."""
+    rng = np.random.default_rng(rng_seed)
+    mols = sysm if isinstance(sysm, list) else [sysm]
+    order = rng.permutation(len(mols)) if isinstance(sysm, list) else np.zeros(64, int)
+    for m in order:
+        mol = mols[int(m)]
+        ns = len(mol.shells)
+        ok = [p for p in rng.permutation(ns * ns) if mol.shells[p // ns].L + mol.shells[p % ns].L <= LSUM_MAX]
+        yield int(m), np.sort(np.array(ok[:8 if isinstance(sysm, list) else 16], dtype=np.int64))
+
+
+def cpu_baseline(sysm, budget_s: float = 25.0):
     """Genuine libgrpp (compiled from its own sources into oracle/_ref) on a bounded sample of the
-    same workload: every k-th ordered shell pair with all centres, OpenMP dynamic over pairs on
-    all host cores.  Falls back to the oracle port when the reference build is absent."""
+    same workload, OpenMP dynamic over ordered pairs on the host threads: chunks of the fixed
+    sample are added (untimed warm-up) until a pass takes about budget/6, then that sample is timed
+    5 times and the median reported.  Falls back to the oracle port when the reference build is
+    absent."""
     import oracle
     lib, kind = oracle.reference(), "reference"
     if lib is None:
         lib, kind = oracle.oracle(), "port"
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    if isinstance(sysm, list):
-        # independent molecules: 8 random ordered shell pairs (all centres) of molecules taken in a
-        # fixed random order until the budget is spent
-        rng = np.random.default_rng(1)
-        ntask, t, used = 0, 0.0, 0
-        for m in rng.permutation(len(sysm)):
-            mol = sysm[int(m)]
-            ns = len(mol.shells)
-            pairs = np.sort(rng.permutation(ns * ns)[:8])
-            t += lib.matrices(mol, nthreads=cores, pairs=pairs)[4]
-            ntask += len(pairs) * len(mol.centers)
-            used += 1
-            if t >= 0.8 * budget_s:
-                break
-        return {"value": ntask / t, "unit": "tasks/s", "cores": cores, "kind": kind,
-                "sample": f"8 random ordered shell pairs x all centres of {used} molecules = {ntask} tasks in "
-                          f"{t:.1f} s"}
-    ns = len(sysm.shells)
-    npairs = ns * ns
-    rng = np.random.default_rng(1)
-    order = rng.permutation(npairs)          # a fixed random sample mixes all (la, lb) classes
-    n = min(npairs, 4 * cores)
-    while True:
-        sample = np.sort(order[:n])
-        t = lib.matrices(sysm, nthreads=cores, pairs=sample)[4]
-        if n >= npairs or t >= 0.4 * budget_s:
+    cores = _threads()
+    mols = sysm if isinstance(sysm, list) else [sysm]
+    chunks, t_warm = [], 0.0
+    for m, pairs in cpu_sample(sysm):
+        t_warm += lib.matrices(mols[m], nthreads=cores, pairs=pairs)[4]
+        chunks.append((m, pairs))
+        if t_warm >= budget_s / 6:
             break
-        n = min(npairs, max(2 * n, int(n * 0.8 * budget_s / max(t, 1e-3))))
-    ntask = n * len(sysm.centers)
-    return {"value": ntask / t, "unit": "tasks/s", "cores": cores, "kind": kind,
-            "sample": f"{n} of {npairs} ordered shell pairs (fixed random sample) x {len(sysm.centers)} centres = "
-                      f"{ntask} tasks in {t:.1f} s"}
+    reps = []
+    for _ in range(5):
+        reps.append(sum(lib.matrices(mols[m], nthreads=cores, pairs=p)[4] for m, p in chunks))
+    t = float(np.median(reps))
+    ntask = sum(len(p) * len(mols[m].centers) for m, p in chunks)
+    out = {"value": ntask / t, "unit": "tasks/s", "cores": cores, "kind": kind,
+           "sample": f"{sum(len(p) for _, p in chunks)} random ordered shell pairs x all centres "
+                     f"({len(chunks)} {'molecules' if len(mols) > 1 else 'chunks'}) = {ntask} tasks; median of 5 "
+                     f"passes after a warm-up pass, {t:.2f} s each",
+           "repetitions_s": reps}
+    out.update(host_info())
+    return out, chunks
+
+
+def percall_leg(sysm, chunks, batched, reps: int = 3):
+    """The reference's own usage pattern through this library's drop-in API: host threads (as
+    OpenMP threads in eval_integrals.c:78-123) each calling libgrpp_full_grpp_integrals for one
+    (shell A, shell B, centre) task of the CPU sample, outputs summed per pair.  Median of ``reps``
+    passes after a warm-up pass; max_abs_vs_batched compares the summed blocks with the batched build."""
+    from concurrent.futures import ThreadPoolExecutor
+    from libgrpp_amd import lib as G
+    from libgrpp_amd.abi import as_dptr, darr
+    L, fac = G.lib(), G.factory()
+    mols = sysm if isinstance(sysm, list) else [sysm]
+    tasks, objs = [], []
+    for m, pairs in chunks:
+        mol = mols[m]
+        sh = [fac.shell(x) for x in mol.shells]
+        gr = [(darr(c), fac.grpp(g)) for c, g in mol.centers]
+        objs.append((sh, [h for _, h in gr]))
+        ns = len(mol.shells)
+        for p in pairs:
+            i, j = divmod(int(p), ns)
+            n = mol.shells[i].cart_size * mol.shells[j].cart_size
+            for cc, h in gr:
+                tasks.append((m, i, j, sh[i], sh[j], h, cc, [np.zeros(n) for _ in range(4)]))
+
+    def run(t):
+        L.libgrpp_full_grpp_integrals(t[3], t[4], t[5], t[6], *[as_dptr(x) for x in t[7]])
+
+    threads = _threads()
+    times = []
+    with ThreadPoolExecutor(threads) as ex:
+        for r in range(reps + 1):
+            t0 = time.perf_counter()
+            list(ex.map(run, tasks))
+            if r:
+                times.append(time.perf_counter() - t0)
+    err = 0.0
+    sums = {}
+    for m, i, j, *_, outs in tasks:
+        sums.setdefault((m, i, j), [0.0, 0.0, 0.0, 0.0])
+        for k in range(4):
+            sums[(m, i, j)][k] = sums[(m, i, j)][k] + outs[k]
+    for (m, i, j), blk in sums.items():
+        mol = mols[m]
+        off = mol.offsets()
+        A, B = mol.shells[i], mol.shells[j]
+        M = batched[m] if isinstance(sysm, list) else batched
+        for k in range(4):
+            ref = M[k][off[i]:off[i] + A.cart_size, off[j]:off[j] + B.cart_size]
+            err = max(err, float(np.abs(blk[k].reshape(A.cart_size, B.cart_size) - ref).max()))
+    for sh, gr in objs:
+        for x in sh:
+            fac.f["delete_shell"](x)
+        for h in gr:
+            fac.free_grpp(h)
+    t = float(np.median(times))
+    return {"value": len(tasks) / t, "unit": "tasks/s", "threads": threads, "tasks": len(tasks),
+            "seconds": t, "max_abs_vs_batched": err,
+            "pattern": "host threads each calling libgrpp_full_grpp_integrals per (A, B, centre) task"}
+
+
+def parity_fields(workload, sysm, out):
+    """per-operator error of the build's matrices against the genuine reference's blocks for the
+    sampled ordered pairs of tests/golden/<workload>_reference.npz (None when absent): max abs,
+    max relative (|x| > 1), largest error / noise floor of the block among elements outside the
+    parity rule (|d| <= 1e-10 or relative 1e-9 where |x| > 1), and the number of block operators
+    outside the rule and beyond 3 floors"""
+    name = {"ragged_molecules": "ragged", "heavy_cluster": "heavy_cluster"}.get(workload)
+    path = os.path.join(ROOT, "tests", "golden", f"{name}_reference.npz") if name else ""
+    if not os.path.exists(path):
+        return None
+    d = np.load(path)
+    mols = sysm if isinstance(sysm, list) else [sysm]
+    res = {}
+    for k, nm in enumerate(["arep", "so_x", "so_y", "so_z"]):
+        res[nm] = {"max_abs": 0.0, "max_rel": 0.0, "worst_error_over_floor": 0.0, "blocks_outside_rule": 0,
+                   "blocks_beyond_3_floors": 0}
+    for b, (m, p) in enumerate(zip(d["molecule"], d["pair"])):
+        mol = mols[int(m)]
+        M = out[int(m)] if isinstance(sysm, list) else out
+        ns = len(mol.shells)
+        i, j = divmod(int(p), ns)
+        off = mol.offsets()
+        o0, o1 = d["offset"][b], d["offset"][b + 1]
+        n = (o1 - o0) // 4
+        for k, nm in enumerate(["arep", "so_x", "so_y", "so_z"]):
+            want = d["ref"][o0 + k * n:o0 + (k + 1) * n]
+            got = M[k][off[i]:off[i] + mol.shells[i].cart_size, off[j]:off[j] + mol.shells[j].cart_size].ravel()
+            dd = np.abs(got - want)
+            big = np.abs(want) > 1.0
+            rule = (dd <= 1e-10) | (big & (dd <= 1e-9 * np.abs(want)))
+            fl = float(d["floor"][b][k])
+            r = res[nm]
+            r["max_abs"] = max(r["max_abs"], float(dd.max()))
+            r["max_rel"] = max(r["max_rel"], float((dd[big] / np.abs(want[big])).max(initial=0.0)))
+            if not rule.all():
+                r["blocks_outside_rule"] += 1
+                q = float((dd[~rule].max() - 1e-10) / max(fl, 1e-300))
+                r["worst_error_over_floor"] = max(r["worst_error_over_floor"], q)
+                if q > 3.0:
+                    r["blocks_beyond_3_floors"] += 1
+    return {"reference": os.path.relpath(path, ROOT), "sampled_blocks": int(len(d["pair"])),
+            "molecules": int(len(np.unique(d["molecule"]))), "per_matrix": res}
 
 
 def main():
@@ -104,14 +272,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="heavy_cluster")
+    ap.add_argument("--workload", default="ragged_molecules")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="time the plain method only")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuning)")
     ap.add_argument("--tuning", default=None,
                     help="fixed launch shapes screen_probe,contract_threads,type1_threads[,stream_layout[,screen_split]] "
                          "(defaults 0 and 4) instead of autotuning")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=25.0)
+    ap.add_argument("--no-percall", action="store_true", help="skip the per-call drop-in leg")
     ap.add_argument("--graph", action="store_true",
                     help="also time the plain build captured once as a hipGraph and replayed (graph_leg)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -266,22 +435,34 @@ def main():
         "kernel_ms": {k.replace("_seconds", ""): 1e3 * v for k, v in mean.items() if k != "device_seconds"},
         # dominant kernel: the Shaw-Hill envelope estimates (FP64 VALU; FLOP equivalents per triple and
         # per fixed-point evaluation as counted by the engine)
-        "roofline": {"bound": "mfma", "kernel": "k_screen", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+        "roofline": {"bound": "valu", "kernel": "k_screen", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_bytes("k_screen", args.workload),
                      "screen_flops": st["screen_flops"], "screen_triples": st["screen_triples"],
                      "screen_steps": st["screen_steps"], "screen_ms": 1e3 * scr,
-                     "radial_product": {"kernel": "k_radial_mfma", "achieved": rad_achieved,
+                     "radial_product": {"kernel": "k_radial_mfma", "bound": "mfma", "achieved": rad_achieved,
                                         "frac": rad_achieved / FP64_PEAK_TFLOPS, "ms": 1e3 * rad,
                                         "flops": st["fp64_flops"], "points": st["n_radial_points"],
                                         "tiles": st["n_tiles"], "traffic": traffic_bytes("k_radial_mfma", args.workload)}},
     }
     if learned_info:
         line["learned_grid"] = learned_info
+    final = out if learned_info else plain_out
+    line["parity"] = parity_fields(args.workload, sysm, final)
+    line["fp64_counters"] = fp64_counters(args.workload, ms / 1e3)
     line["autotune"] = tuning_plain
     if graph_info:
         line["graph_leg"] = graph_info
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(sysm, args.cpu_budget)
+        line["cpu_baseline"], _ = cpu_baseline(sysm, args.cpu_budget)
+    if not args.no_percall:
+        chunks, ntask = [], 0
+        mols = sysm if molecules else [sysm]
+        for m, pairs in cpu_sample(sysm):
+            chunks.append((m, pairs))
+            ntask += len(pairs) * len(mols[m].centers)
+            if ntask >= 1024:
+                break
+        line["percall"] = percall_leg(sysm, chunks, final)
     print(json.dumps(line))
 
 
