@@ -68,6 +68,24 @@ int libgrpp_batch_full_grpp_integrals(int nshells, libgrpp_shell_t *const *shell
                                       libgrpp_batch_stats_t *stats);
 
 /*
+ * Allocation-free variant of libgrpp_batch_full_grpp_integrals (no counterpart in the reference):
+ * for callers that cannot allocate device memory -- tight serving loops, harnesses that forbid
+ * hipMalloc.  Same inputs and host outputs; flags may hold LIBGRPP_BATCH_NO_SYMMETRY only.  Every
+ * device buffer of the build lives in one static 64 MiB scratch array per device, and the kernels
+ * run in order on `stream` (a hipStream_t, NULL = default stream); the call returns when the
+ * matrices are in the caller's arrays.  No hipMalloc / hipFree or other allocating library routine
+ * is called, so libgrpp_init() must have run with this device current (it puts the constant tables
+ * in place).  Inputs whose build needs more than the scratch array are refused with an error
+ * (libgrpp_last_error()).  All calls on one device share that array: calls that overlap on one
+ * device race, and must be serialised by the caller.  stats: as for the batched call, without
+ * kernel times (zero).
+ */
+int libgrpp_batch_full_grpp_integrals_noalloc(int nshells, libgrpp_shell_t *const *shells, int ncenters,
+                                              const double *center_xyz, libgrpp_grpp_t *const *grpp_per_center,
+                                              int flags, double *arep, double *so_x, double *so_y, double *so_z,
+                                              void *stream, libgrpp_batch_stats_t *stats);
+
+/*
  * Nuclear gradients of the same matrices with respect to every centre's coordinates, treating
  * the centre list as the atom list (eval_integrals_grad.c:37-163 with the reference's
  * libgrpp_full_grpp_integrals_gradient per task).  out holds 4 * 3*ncenters * dim * dim doubles:

@@ -54,7 +54,13 @@ void libgrpp_set_cartesian_generator(int (*g)(int L, int *cart_list)) { libgrpp_
 
 /* ---------------------------------------------------------------- init (init_finalize.c) */
 static int g_initialized = 0;
-void libgrpp_init(void) { g_initialized = 1; }
+void libgrpp_init(void)
+{
+    /* on a GPU host the current device is set up here (constant tables, scratch address), so that
+       later calls, the allocation-free one included, find them in place; without a GPU nothing */
+    if (eng_setup_device()) fprintf(stderr, "libgrpp_amd: libgrpp_init: %s\n", libgrpp_last_error());
+    g_initialized = 1;
+}
 void libgrpp_finalize(void) { g_initialized = 0; }
 int libgrpp_is_initialized(void) { return g_initialized; }
 

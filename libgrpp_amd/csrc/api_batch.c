@@ -340,8 +340,18 @@ static eng_center_t *make_centers(int ncenters, const double *xyz, libgrpp_grpp_
     return cs;
 }
 
+static int batch_plan_create(int nshells, libgrpp_shell_t *const *shells, int ncenters, const double *xyz,
+                             libgrpp_grpp_t *const *grpps, int flags, int eng_extra, libgrpp_batch_plan_t **plan);
+
 int libgrpp_batch_plan_create(int nshells, libgrpp_shell_t *const *shells, int ncenters, const double *xyz,
                               libgrpp_grpp_t *const *grpps, int flags, libgrpp_batch_plan_t **plan)
+{
+    return batch_plan_create(nshells, shells, ncenters, xyz, grpps, flags, 0, plan);
+}
+
+/* eng_extra: engine flags beyond those the public flags map to (ENG_F_ARENA) */
+static int batch_plan_create(int nshells, libgrpp_shell_t *const *shells, int ncenters, const double *xyz,
+                             libgrpp_grpp_t *const *grpps, int flags, int eng_extra, libgrpp_batch_plan_t **plan)
 {
     *plan = NULL;
     if (!check_flags("batch_plan_create", flags)) return 2;
@@ -373,7 +383,7 @@ int libgrpp_batch_plan_create(int nshells, libgrpp_shell_t *const *shells, int n
     eng_plan_t *eng = NULL;
     long long n2 = (long long) dim * dim;
     int eflags = ((flags & LIBGRPP_BATCH_RECORD_LEVELS) ? ENG_F_LEVELS : 0) |
-                 ((flags & LIBGRPP_BATCH_GRAPH) ? ENG_F_GRAPH : 0);
+                 ((flags & LIBGRPP_BATCH_GRAPH) ? ENG_F_GRAPH : 0) | eng_extra;
     int rc = eng_plan_create(nshells, shells, ncenters, cs, nidx, cidx, nit, it, n2, n2, ENG_ALL, eflags, &eng);
     free(off);
     free(cidx);
@@ -598,6 +608,31 @@ int libgrpp_batch_full_grpp_integrals(int nshells, libgrpp_shell_t *const *shell
     if (rc) return rc;
     rc = libgrpp_batch_plan_execute(p, NULL);
     if (!rc) rc = libgrpp_batch_plan_download(p, arep, so_x, so_y, so_z);
+    if (!rc && stats) libgrpp_batch_plan_stats(p, stats);
+    libgrpp_batch_plan_destroy(p);
+    return rc;
+}
+
+/*
+ * The same matrices with no device allocation: every device buffer of the build is carved from
+ * the engine's static scratch array of this device (ENG_SCRATCH_BYTES), and the kernels run in
+ * order on the caller's stream with no stream or event of their own; the four matrices are then
+ * copied into the caller's host arrays.  Inputs that need more scratch than that are refused.
+ */
+int libgrpp_batch_full_grpp_integrals_noalloc(int nshells, libgrpp_shell_t *const *shells, int ncenters,
+                                              const double *xyz, libgrpp_grpp_t *const *grpps, int flags,
+                                              double *arep, double *so_x, double *so_y, double *so_z, void *stream,
+                                              libgrpp_batch_stats_t *stats)
+{
+    if (flags & (LIBGRPP_BATCH_GRAPH | LIBGRPP_BATCH_RECORD_LEVELS)) {
+        eng_set_error("batch_full_grpp_integrals_noalloc: GRAPH and RECORD_LEVELS are not available without allocation");
+        return 2;
+    }
+    libgrpp_batch_plan_t *p = NULL;
+    int rc = batch_plan_create(nshells, shells, ncenters, xyz, grpps, flags, ENG_F_ARENA, &p);
+    if (rc) return rc;
+    rc = libgrpp_batch_plan_execute(p, stream);
+    if (!rc) rc = eng_plan_download_ops(p->eng, arep, so_x, so_y, so_z);
     if (!rc && stats) libgrpp_batch_plan_stats(p, stats);
     libgrpp_batch_plan_destroy(p);
     return rc;

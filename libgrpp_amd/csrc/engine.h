@@ -55,7 +55,11 @@ typedef struct {
 } eng_tuning_t;
 
 /* flags */
-enum { ENG_F_STATS = 2, ENG_F_LEVELS = 4, ENG_F_GRAPH = 8 };
+enum { ENG_F_STATS = 2, ENG_F_LEVELS = 4, ENG_F_GRAPH = 8,
+       ENG_F_ARENA = 16 /* every device buffer carved from the device's static scratch array, no
+                           stream or event of the plan's own: the build runs in order on the
+                           caller's stream (the allocation-free entry point) */ };
+#define ENG_SCRATCH_BYTES (64ull << 20)   /* size of that static scratch array per device */
 #define ENG_NFEAT 12   /* level-predictor features per radial row */
 
 int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int ncenter, const eng_center_t *centers,
@@ -63,6 +67,8 @@ int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int ncenter, con
                     long long op_stride, int parts, int flags, eng_plan_t **plan);
 int eng_plan_execute(eng_plan_t *plan, void *stream);
 int eng_plan_download(eng_plan_t *plan, double *out);   /* out_len * 4 doubles */
+/* the four operators' first out_len doubles straight into four host arrays */
+int eng_plan_download_ops(eng_plan_t *plan, double *o0, double *o1, double *o2, double *o3);
 double *eng_plan_device_output(eng_plan_t *plan);
 void eng_plan_destroy(eng_plan_t *plan);
 int eng_plan_stats(const eng_plan_t *plan, libgrpp_batch_stats_t *stats);
@@ -80,6 +86,7 @@ int eng_run(int nshell, libgrpp_shell_t *const *shells, int ncenter, const eng_c
 
 void eng_set_error(const char *fmt, ...);
 int eng_device_ok(void);
+int eng_setup_device(void);   /* tables, LDS limits, scratch address on the current device (if any) */
 
 #ifdef __cplusplus
 }

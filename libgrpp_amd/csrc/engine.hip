@@ -1761,14 +1761,53 @@ static int raise_lds_limits(void)
     return 0;
 }
 
-static int device_tables(int Lmax, int nmax, DevTables *out)
+// Scratch memory of the allocation-free entry point (libgrpp_batch_full_grpp_integrals_noalloc):
+// one static device array per GPU that loads this library, held until the process ends.  In host
+// code its name is only a stand-in: the device address is looked up once per device, with that
+// device current, and kept in g_scratch; kernels and copies are given that address.  Every call on
+// one device shares the array, so calls that overlap on one device race (the entry point
+// documents it); nothing in it is assumed to start at zero.
+static __device__ __align__(256) unsigned char scratch[ENG_SCRATCH_BYTES];
+static std::map<int, unsigned char *> g_scratch;   // device -> address of its scratch array
+
+// under g_tab_mutex, with device dev current
+static int lookup_scratch(int dev)
+{
+    if (g_scratch.count(dev)) return 0;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        eng_set_error("scratch array of device %d: hipGetSymbolAddress failed: %s", dev, hipGetErrorString(e));
+        return 1;
+    }
+    g_scratch[dev] = (unsigned char *) ptr;
+    return 0;
+}
+
+static int scratch_base(int dev, unsigned char **out)
+{
+    std::lock_guard<std::mutex> lk(g_tab_mutex);
+    if (lookup_scratch(dev)) return 1;
+    *out = g_scratch[dev];
+    return 0;
+}
+
+// no_alloc: the tables must already be on the device at sufficient size (set up by
+// eng_setup_device); refuses instead of uploading
+static int device_tables(int Lmax, int nmax, DevTables *out, bool no_alloc = false)
 {
     std::lock_guard<std::mutex> lk(g_tab_mutex);
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     DeviceTables &D = g_dev_tables[dev];
+    if (no_alloc && (!D.ok || Lmax > D.Lmax || nmax > D.nmax)) {
+        eng_set_error("the allocation-free entry point needs the device tables of device %d in place: call "
+                      "libgrpp_init() with that device current before the first call", dev);
+        return 1;
+    }
     if (!D.ok) {
         if (raise_lds_limits()) return 1;
+        if (lookup_scratch(dev)) return 1;
         const HostTables &H = host_tables();
         double *p;
         int *ip;
@@ -1830,6 +1869,17 @@ static int device_tables(int Lmax, int nmax, DevTables *out)
     return 0;
 }
 
+// Library set-up on the current device (libgrpp_init): the constant tables at the largest
+// angular momenta the engine supports, the dynamic-LDS limits and the scratch array's address, so
+// that the allocation-free entry point finds everything in place.  Without a device: nothing.
+extern "C" int eng_setup_device(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 0;
+    DevTables t;
+    return device_tables(LMAX_POT, LMAX_SHELL, &t);
+}
+
 extern "C" int eng_device_ok(void)
 {
     int n = 0;
@@ -1848,6 +1898,9 @@ struct eng_plan {
     DevJob J{};
     int device = 0;
     std::vector<std::pair<void *, size_t>> bufs;   // device buffers and their pooled sizes
+    unsigned char *arena = nullptr;               // ENG_F_ARENA: buffers carved from the scratch array
+    size_t arena_used = 0;
+    bool serial = false;                          // one stream (the caller's), no events
     double *d_out = nullptr, *d_F = nullptr, *d_wU = nullptr, *d_Q = nullptr, *d_P = nullptr, *d_OCV = nullptr,
            *d_delta = nullptr, *d_scr = nullptr, *d_Qc = nullptr;
     TabItem *d_tab = nullptr;
@@ -1956,6 +2009,17 @@ static int dalloc(eng_plan *P, T **p, size_t n)
 {
     const size_t bytes = (std::max<size_t>(1, n) * sizeof(T) + 255) & ~(size_t) 255;
     void *q = nullptr;
+    if (P->arena) {
+        if (P->arena_used + bytes > ENG_SCRATCH_BYTES) {
+            eng_set_error("input too large for the allocation-free entry point: its buffers exceed the %llu-byte "
+                          "scratch array (already %zu bytes, next buffer %zu)",
+                          (unsigned long long) ENG_SCRATCH_BYTES, P->arena_used, bytes);
+            return 1;
+        }
+        *p = (T *) (P->arena + P->arena_used);
+        P->arena_used += bytes;
+        return 0;
+    }
     if (pool_alloc(P->device, bytes, &q)) return 1;
     *p = (T *) q;
     P->bufs.push_back({q, bytes});
@@ -1985,6 +2049,14 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     if (!eng_device_ok()) return 1;
     eng_plan *P = new eng_plan();
     HIPCHK(hipGetDevice(&P->device));
+    if (flags & ENG_F_ARENA) {
+        if ((flags & ENG_F_GRAPH) || scratch_base(P->device, &P->arena)) {
+            if (flags & ENG_F_GRAPH) eng_set_error("the allocation-free build does not capture a graph");
+            delete P;
+            return 1;
+        }
+        P->serial = true;
+    }
     P->nshell = nshell;
     P->ncenter = ncenter;
     P->parts = parts;
@@ -2322,7 +2394,7 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     }
 
     // ---------------- upload ----------------
-    if (device_tables(Lmax_pot, lmax_shell, &P->T)) {
+    if (device_tables(Lmax_pot, lmax_shell, &P->T, P->arena != nullptr)) {
         delete P;
         return 1;
     }
@@ -2523,10 +2595,12 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     J.OCV = P->d_OCV;
     J.nshell = nshell;   // lookup tables are indexed by basis shell; outercore shells follow them in sh
     J.op_stride = op_stride;
-    for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&P->ev[i]));
-    for (int i = 0; i < 3; i++) HIPCHK(hipEventCreate(&P->evs[i]));
-    HIPCHK(hipEventCreate(&P->evt1));
-    HIPCHK(hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking));
+    if (!P->serial) {
+        for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&P->ev[i]));
+        for (int i = 0; i < 3; i++) HIPCHK(hipEventCreate(&P->evs[i]));
+        HIPCHK(hipEventCreate(&P->evt1));
+        HIPCHK(hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking));
+    }
     if (flags & ENG_F_GRAPH) {
         HIPCHK(hipStreamCreateWithFlags(&P->cap, hipStreamNonBlocking));
         for (int i = 0; i < 2; i++) HIPCHK(hipEventCreate(&P->gev[i]));
@@ -2566,26 +2640,32 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
     // Events: caller 0 start = 1 | screening | 2 | radial | 3 | canonical | 4 ... 6 | contraction,
     // reduction | 7; s0 | tables | 5 | projectors, outercore | s1 on the tables' stream;
     // side t1 | type-1 | s2.
+    // serial plans (ENG_F_ARENA) run every kernel in this order on the caller's stream, which
+    // orders them as the events below would; they own no stream or event
+    const bool ser = P->serial;
+    auto rec = [&](hipEvent_t e, hipStream_t s) { return ser ? hipSuccess : hipEventRecord(e, s); };
+    auto wait = [&](hipStream_t s, hipEvent_t e) { return ser ? hipSuccess : hipStreamWaitEvent(s, e, 0); };
+    hipStream_t side = ser ? st : P->side;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, P->d_counters, 8);
-    HIPCHK(hipEventRecord(P->ev[0], st));
-    HIPCHK(hipStreamWaitEvent(P->side, P->ev[0], 0));
+    HIPCHK(rec(P->ev[0], st));
+    HIPCHK(wait(side, P->ev[0]));
     const bool front = P->tune.layout == 1;
-    hipStream_t sd = front ? st : P->side;
+    hipStream_t sd = front ? st : side;
     if (front) {   // type-1 needs no table: the side stream starts with it
-        HIPCHK(hipEventRecord(P->evt1, P->side));
+        HIPCHK(rec(P->evt1, side));
         if (P->n_parts > 0)
             hipLaunchKernelGGL(k_type1, dim3(P->n_parts), dim3(P->tune.type1_threads),
-                               (size_t) (P->t1_scratch + P->t1_acc) * sizeof(double), P->side, P->d_parts_run,
+                               (size_t) (P->t1_scratch + P->t1_acc) * sizeof(double), side, P->d_parts_run,
                                P->d_items, P->J, P->T, P->d_partial, P->parts, P->t1_scratch);
-        HIPCHK(hipEventRecord(P->evs[2], P->side));
+        HIPCHK(rec(P->evs[2], side));
     }
-    HIPCHK(hipEventRecord(P->evs[0], sd));
+    HIPCHK(rec(P->evs[0], sd));
     // grid tables first: the radial product waits for them (event ev[5]); the screening does not
     if (P->n_tab > 0) {
         dim3 g(NPTS / 256, P->n_tab);
         hipLaunchKernelGGL(k_tab_F, g, dim3(256), 0, sd, P->d_tab, P->J, P->T, P->d_F);
     }
-    HIPCHK(hipEventRecord(P->ev[5], sd));
+    HIPCHK(rec(P->ev[5], sd));
     if (P->n_proj > 0) hipLaunchKernelGGL(k_proj, dim3(P->n_proj), dim3(256), 0, sd, P->d_proj, P->J, P->T, P->d_P);
     if (P->n_delta_centers > 0)
         hipLaunchKernelGGL(k_delta, dim3(P->n_delta_centers), dim3(64), 0, sd, P->d_delta_centers,
@@ -2593,16 +2673,16 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
     if (P->n_ocv > 0)
         hipLaunchKernelGGL(k_ocv, dim3(P->n_ocv), dim3(256), (size_t) (P->ocv_scratch + 2 * P->ocv_nel) * sizeof(double),
                            sd, P->d_ocvi, P->J, P->T, P->d_OCV, P->ocv_scratch, P->ocv_nel);
-    HIPCHK(hipEventRecord(P->evs[1], sd));
+    HIPCHK(rec(P->evs[1], sd));
     if (!front) {
-        HIPCHK(hipEventRecord(P->evt1, sd));
+        HIPCHK(rec(P->evt1, sd));
         if (P->n_parts > 0)
             hipLaunchKernelGGL(k_type1, dim3(P->n_parts), dim3(P->tune.type1_threads),
                                (size_t) (P->t1_scratch + P->t1_acc) * sizeof(double),
                                sd, P->d_parts_run, P->d_items, P->J, P->T, P->d_partial, P->parts, P->t1_scratch);
-        HIPCHK(hipEventRecord(P->evs[2], sd));
+        HIPCHK(rec(P->evs[2], sd));
     }
-    HIPCHK(hipEventRecord(P->ev[1], st));
+    HIPCHK(rec(P->ev[1], st));
     if (P->n_tiles > 0) {
         const int sp = P->tune.screen_probe, ss = P->tune.screen_split;
         auto screen = ss == 16 ? (sp == 1 ? k_screen<1, 16> : sp == 2 ? k_screen<2, 16> : k_screen<4, 16>)
@@ -2611,8 +2691,8 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
         hipLaunchKernelGGL(screen, dim3(ss * P->n_tiles), dim3(64), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->J, P->T,
                            P->d_scr, P->d_counters);
     }
-    HIPCHK(hipStreamWaitEvent(st, P->ev[5], 0));   // grid tables of the side stream
-    HIPCHK(hipEventRecord(P->ev[2], st));
+    HIPCHK(wait(st, P->ev[5]));   // grid tables of the side stream
+    HIPCHK(rec(P->ev[2], st));
     if (P->n_tiles > 0) {
         if (P->radial_mfma && P->d_lev)
             hipLaunchKernelGGL(k_radial_mfma<true>, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles_radial, P->d_jobs,
@@ -2627,18 +2707,18 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
                                P->T.rpow, P->d_wU, P->d_Q, P->min_level, P->tol, P->d_counters, P->d_scr, P->J.sh,
                                P->d_rowpos, P->d_lev);
     }
-    HIPCHK(hipEventRecord(P->ev[3], st));
+    HIPCHK(rec(P->ev[3], st));
     if (P->n_tiles > 0)
         hipLaunchKernelGGL(k_canon, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->J, P->d_Q,
                            P->d_Qc);
-    HIPCHK(hipEventRecord(P->ev[4], st));
-    HIPCHK(hipStreamWaitEvent(st, P->evs[1], 0));   // the contraction reads projectors and outercore vectors
-    HIPCHK(hipEventRecord(P->ev[6], st));
+    HIPCHK(rec(P->ev[4], st));
+    HIPCHK(wait(st, P->evs[1]));   // the contraction reads projectors and outercore vectors
+    HIPCHK(rec(P->ev[6], st));
     if (P->n_parts > 0)
         hipLaunchKernelGGL(contract_kernel(P->tune.contract_threads), dim3(P->n_parts), dim3(P->tune.contract_threads),
                            (size_t) (P->ct_qs + P->ct_ts) * sizeof(double), st, P->d_parts_run, P->d_items, P->J, P->T,
                            P->d_partial_ct, P->parts, P->nm_max, P->ct_qs, P->ct_ts);
-    HIPCHK(hipStreamWaitEvent(st, P->evs[2], 0));   // join: the reduction reads the type-1 slices
+    HIPCHK(wait(st, P->evs[2]));   // join: the reduction reads the type-1 slices
     if (P->n_items > 0)   // also writes the zero blocks of items without centres
         hipLaunchKernelGGL(k_contract_reduce, dim3(P->n_items), dim3(256), 0, st, P->d_items, P->d_item_part0,
                            P->d_parts, P->J, P->d_partial, P->d_partial_ct, P->d_out);
@@ -2675,6 +2755,13 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
         HIPCHK(hipEventElapsedTime(&ms, P->gev[0], P->gev[1]));
         P->last_seconds = ms * 1e-3;
         P->last_radial_seconds = 0.0;
+    } else if (P->serial) {
+        const int rc = enqueue_build(P, st);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(st));
+        for (int i = 0; i < 7; i++) P->phase_seconds[i] = 0.0;   // no events: not timed
+        P->last_seconds = 0.0;
+        P->last_radial_seconds = 0.0;
     } else {
         const int rc = enqueue_build(P, st);
         if (rc) return rc;
@@ -2683,7 +2770,7 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
         const int rt = eager_phase_times(P);
         if (rt) return rt;
     }
-    if (!P->radial_order_fresh && P->n_tiles > 0 && P->radial_mfma) {
+    if (!P->serial && !P->radial_order_fresh && P->n_tiles > 0 && P->radial_mfma) {
         // the tiles that ran to the highest levels go first in the next builds (no result depends
         // on the order); refreshed after every change of the row order
         std::vector<int> lev(P->n_tiles);
@@ -2728,12 +2815,22 @@ extern "C" int eng_plan_download(eng_plan_t *P, double *out)
     return 0;
 }
 
+extern "C" int eng_plan_download_ops(eng_plan_t *P, double *o0, double *o1, double *o2, double *o3)
+{
+    double *o[4] = {o0, o1, o2, o3};
+    for (int k = 0; k < 4; k++)
+        HIPCHK(hipMemcpy(o[k], P->d_out + (size_t) k * P->op_stride, (size_t) P->out_len * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    return 0;
+}
+
 extern "C" double *eng_plan_device_output(eng_plan_t *P) { return P->d_out; }
 
 extern "C" void eng_plan_destroy(eng_plan_t *P)
 {
     if (!P) return;
-    for (auto &b : P->bufs) pool_free(P->device, b.first, b.second);
+    if (!P->arena)
+        for (auto &b : P->bufs) pool_free(P->device, b.first, b.second);
     for (int i = 0; i < 8; i++)
         if (P->ev[i]) (void) hipEventDestroy(P->ev[i]);
     for (int i = 0; i < 3; i++)

@@ -62,6 +62,10 @@ def lib() -> C.CDLL:
         L.libgrpp_batch_full_grpp_integrals.restype = C.c_int
         L.libgrpp_batch_full_grpp_integrals.argtypes = [C.c_int, C.POINTER(PSHELL), C.c_int, dptr, C.POINTER(PGRPP),
                                                         C.c_int, dptr, dptr, dptr, dptr, C.POINTER(Stats)]
+        L.libgrpp_batch_full_grpp_integrals_noalloc.restype = C.c_int
+        L.libgrpp_batch_full_grpp_integrals_noalloc.argtypes = [C.c_int, C.POINTER(PSHELL), C.c_int, dptr,
+                                                                C.POINTER(PGRPP), C.c_int, dptr, dptr, dptr, dptr,
+                                                                C.c_void_p, C.POINTER(Stats)]
         L.libgrpp_batch_full_grpp_gradients.restype = C.c_int
         L.libgrpp_batch_full_grpp_gradients.argtypes = [C.c_int, C.POINTER(PSHELL), C.c_int, dptr, C.POINTER(PGRPP),
                                                         C.c_int, dptr, C.POINTER(Stats)]
@@ -133,6 +137,23 @@ def batch_matrices(sysm: System, flags: int = 0):
         rc = lib().libgrpp_batch_full_grpp_integrals(len(sysm.shells), o.shells, len(sysm.centers), as_dptr(o.xyz),
                                                      o.grpps, flags, *[as_dptr(m) for m in out], C.byref(st))
         _check(rc, "libgrpp_batch_full_grpp_integrals")
+    finally:
+        o.close()
+    return out[0], out[1], out[2], out[3], st.as_dict()
+
+
+def batch_matrices_noalloc(sysm: System, flags: int = 0, stream=None):
+    """libgrpp_batch_full_grpp_integrals_noalloc: the same matrices with no device allocation
+    (buffers in the library's per-device scratch array; inputs too large for it raise)."""
+    o = _Objects(sysm)
+    dim = sysm.dim
+    out = [np.zeros((dim, dim)) for _ in range(4)]
+    st = Stats()
+    try:
+        rc = lib().libgrpp_batch_full_grpp_integrals_noalloc(len(sysm.shells), o.shells, len(sysm.centers),
+                                                             as_dptr(o.xyz), o.grpps, flags,
+                                                             *[as_dptr(m) for m in out], stream, C.byref(st))
+        _check(rc, "libgrpp_batch_full_grpp_integrals_noalloc")
     finally:
         o.close()
     return out[0], out[1], out[2], out[3], st.as_dict()
