@@ -311,7 +311,10 @@ __device__ __forceinline__ long long canon_index(const RowDesc *R, const DShell 
     const int cn = a_first ? dev_canon_nA(L, la, lb, ra.lam, rb.lam, N) : dev_canon_nA_max(L, la, lb, ra.lam, rb.lam, N);
     const int r2 = ra.lidx - dev_kcol(L, ra.n, ra.lam) + dev_kcol(L, cn, ra.lam);
     const int c2 = rb.lidx - dev_kcol(L, rb.n, rb.lam) + dev_kcol(L, N - cn, rb.lam);
-    return (long long) pos[jb.rows + r2] * jb.M + pos[jb.rows + c2];
+    (void) pos;
+    // the estimate table is kept once, in the upper triangle of the LOGICAL row order (the order
+    // in which the screening walks its tiles), whatever the physical order of the rows
+    return (long long) min(r2, c2) * jb.M + max(r2, c2);
 }
 
 
@@ -518,7 +521,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
     // full-cost ones first and the ones that vanish analytically (K_l(0) = 0 for l >= 1 on a shell
     // sitting on the centre) last, so lanes run entries of like cost side by side.
     __shared__ int list[RT * RT / SCR_SPLIT];
-    __shared__ unsigned char settled[RT * RT / SCR_SPLIT];   // entries whose probe found a failing triple
+    // per tile position e of the wave: 0 nothing to store (outside the table, lower triangle or not
+    // a canonical split), 1 estimate pending, 2 vanishes analytically, 3 a triple failed (NaN),
+    // 4 evaluated and stored
+    __shared__ unsigned char st[RT * RT / SCR_SPLIT];
     __shared__ int n_full, n_zero;
     const TileItem t = tiles[blockIdx.x / SCR_SPLIT];
     const QJob jb = jobs[t.job];
@@ -537,6 +543,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
     auto phys = [&](int l) { return J.rowpos[jb.rows + l]; };
     for (int e = threadIdx.x; e < NE; e += blockDim.x) {
         const int lrow = t.ti * RT + (e / RT) * SCR_SPLIT + sub, lcol = t.tj * RT + e % RT;
+        st[e] = 0;
         if (lrow >= jb.M || lcol >= jb.M) continue;
         if (t.ti == t.tj && lrow > lcol) continue;
         const RowDesc ra = R[phys(lrow)], rb = R[phys(lcol)];
@@ -547,19 +554,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
         const bool a_on = (c.x - A.x) * (c.x - A.x) + (c.y - A.y) * (c.y - A.y) + (c.z - A.z) * (c.z - A.z) == 0.0;
         const bool b_on = (c.x - B.x) * (c.x - B.x) + (c.y - B.y) * (c.y - B.y) + (c.z - B.z) * (c.z - B.z) == 0.0;
         const bool zero = (a_on && ra.lam >= 1) || (b_on && rb.lam >= 1);
-        if (zero) list[NE - 1 - atomicAdd(&n_zero, 1)] = e;
-        else list[atomicAdd(&n_full, 1)] = e;
+        // analytically vanishing entries: every triple is zero, so the estimate is 0 (screened out)
+        if (zero) {
+            list[NE - 1 - atomicAdd(&n_zero, 1)] = e;
+            st[e] = 2;
+        } else {
+            list[atomicAdd(&n_full, 1)] = e;
+            st[e] = 1;
+        }
     }
     __syncthreads();
-    const int nf = n_full, nz = n_zero;
+    const int nf = n_full;
     unsigned ntrip = 0, nstep = 0;   // primitive triples and fixed-point evaluations (work accounting)
-    // analytically vanishing entries: every triple is zero, so the estimate is 0 (screened out)
-    for (int q = threadIdx.x; q < nz; q += blockDim.x) {
-        const int e = list[NE - nz + q];
-        const int row = phys(t.ti * RT + (e / RT) * SCR_SPLIT + sub), col = phys(t.tj * RT + e % RT);
-        scr[jb.qoff + (long long) row * jb.M + col] = 0.0;
-        scr[jb.qoff + (long long) col * jb.M + row] = 0.0;
-    }
     // an entry with a triple that fails the fixed point has no estimate (NaN), whatever its other
     // triples give; the triples most likely to fail come first in the probe order, so a probe of
     // the first SCR_PROBE triples of 64 / SCR_PROBE entries at once settles most entries of a workload
@@ -605,28 +611,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
                 ntrip++;
                 nstep += rsteps;
             }
-            if (k == 0) {
-                scr[jb.qoff + (long long) row * jb.M + col] = NaN;
-                scr[jb.qoff + (long long) col * jb.M + row] = NaN;
-            }
+            if (k == 0) st[list[qi]] = 3;
         }
-        if (k == 0 && qi < nf) settled[qi] = gbits ? 1 : 0;
     }
     __syncthreads();
     // the entries the probe did not settle: the whole wave per entry
 #pragma unroll 1
     for (int q = 0; q < nf; q++) {
-        if (settled[q]) continue;
+        const int e = list[q];
+        if (st[e] == 3) continue;
         int row, col;
         RowDesc ra, rb;
         DShell A, B;
         double CA, CB;
         entry(q, row, col, ra, rb, A, B, CA, CB);
         const double v = screen_entry_wave(J, T, A, B, CA, CB, P, ra.lam, rb.lam, ra.n + rb.n, ntrip, nstep);
-        if (threadIdx.x == 0) {
-            scr[jb.qoff + (long long) row * jb.M + col] = v;
-            scr[jb.qoff + (long long) col * jb.M + row] = v;
-        }
+        if (threadIdx.x == 0)
+            scr[jb.qoff + (long long) (t.ti * RT + (e / RT) * SCR_SPLIT + sub) * jb.M + t.tj * RT + e % RT] = v;
+    }
+    __syncthreads();
+    // the failed (NaN) and vanishing (0) estimates in tile order: a row's 64 columns are 64
+    // consecutive doubles of the logical upper triangle, one store per wave
+    for (int e = threadIdx.x; e < NE; e += blockDim.x) {
+        const unsigned char v = st[e];
+        if (v == 2 || v == 3)
+            scr[jb.qoff + (long long) (t.ti * RT + (e / RT) * SCR_SPLIT + sub) * jb.M + t.tj * RT + e % RT] =
+                v == 2 ? 0.0 : NaN;
     }
     if (counters) {
         unsigned long long a = ntrip, b = nstep;
