@@ -131,7 +131,8 @@ typedef struct {
     int contract_threads;     /* 64, 128 or 256 */
     int type1_threads;        /* 64, 128 or 256 */
     int stream_layout;        /* 0: grid tables, projectors and type-1 beside the screening on a second
-                                 stream; 1: tables and projectors ahead of it, type-1 beside it */
+                                 stream; 1: tables and projectors ahead of it, type-1 beside it;
+                                 2: every kernel in order on the caller's stream */
     int screen_split;         /* 4, 8 or 16 screening waves per 64 x 64 radial tile; 0 in
                                  libgrpp_batch_plan_set_tuning keeps the plan's current split (the
                                  plan's default: 16 when it has fewer than 8192 radial tiles, else 4).

@@ -548,13 +548,13 @@ int libgrpp_batch_plan_autotune(libgrpp_batch_plan_t *p, int repeats, libgrpp_ba
     int rc = time_builds(p, repeats, 0, &before);
     if (rc) return rc;
     /* one parameter at a time, each timed by the phase it acts on */
-    static const int probes[3] = {1, 2, 4}, splits[3] = {4, 8, 16}, threads[3] = {64, 128, 256}, layouts[2] = {0, 1};
+    static const int probes[3] = {1, 2, 4}, splits[3] = {4, 8, 16}, threads[3] = {64, 128, 256}, layouts[3] = {0, 1, 2};
     /* par: 0 probe width, 1 contraction threads, 2 type-1 threads, 3 stream layout, 4 screening split */
     for (int par = 0; par < 5; par++) {
         int *field = par == 0 ? &t.screen_probe : par == 1 ? &t.contract_threads : par == 2 ? &t.type1_threads
                    : par == 3 ? &t.layout : &t.screen_split;
         const int *cand = par == 0 ? probes : par == 3 ? layouts : par == 4 ? splits : threads;
-        const int ncand = par == 3 ? 2 : 3;
+        const int ncand = 3;
         int best_v = *field;
         double best_t = 1e300;
         for (int c = 0; c < ncand; c++) {

@@ -50,7 +50,8 @@ typedef struct {
     int contract_threads;   /* k_contract workgroup: 64, 128 or 256 threads */
     int type1_threads;      /* k_type1 workgroup: 64, 128 or 256 threads */
     int layout;             /* 0: tables, projectors, type-1 on the side stream; 1: tables and projectors
-                               ahead of the screening, type-1 alone on the side stream */
+                               ahead of the screening, type-1 alone on the side stream; 2: every kernel
+                               in order on the caller's stream */
     int screen_split;       /* k_screen waves per 64 x 64 radial tile: 4, 8 or 16 */
 } eng_tuning_t;
 

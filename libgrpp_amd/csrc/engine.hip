@@ -1148,6 +1148,36 @@ __device__ void mmd_E_dir(double *E, int la, int lb, double q, double K, double 
 #undef EI
 }
 
+// The t = 0 column E^{ij}_0 alone, which is all an r^0 Gaussian term needs (its integral is
+// (pi/q)^{3/2} E^x_0 E^y_0 E^z_0): E^{i-1,j}_1 is formed on the fly from the t = 0 entries by the
+// same expression mmd_E_dir evaluates, so the values are those of the full table, at
+// (la+1)(lb+1) entries per direction instead of (la+1)(lb+1)(la+lb+1).
+__device__ void mmd_E0_dir(double *E, int la, int lb, double q, double K, double XQA, double XQB)
+{
+    const double f = 1.0 / (2.0 * q * 1);
+#define E0(i, j) E[(i) * (lb + 1) + (j)]
+    for (int i = 0; i <= la; i++)
+        for (int j = 0; j <= lb; j++) {
+            if (i == 0 && j == 0) {
+                E0(0, 0) = K;
+            } else if (i == 1 && j == 0) {
+                E0(1, 0) = XQA * E0(0, 0);
+            } else if (i == 0 && j == 1) {
+                E0(0, 1) = XQB * E0(0, 0);
+            } else if (i == 0) {
+                double e1 = 0.0;   // E^{0,j-1}_1
+                if (j - 1 > 0) e1 += f * (j - 1) * E0(0, j - 2);
+                E0(0, j) = XQB * E0(0, j - 1) + e1;
+            } else {
+                double e1 = 0.0;   // E^{i-1,j}_1
+                if (i - 1 > 0) e1 += f * (i - 1) * E0(i - 2, j);
+                if (j > 0) e1 += f * j * E0(i - 1, j - 1);
+                E0(i, j) = XQA * E0(i - 1, j) + e1;
+            }
+        }
+#undef E0
+}
+
 __device__ __forceinline__ int simplex_idx(int t, int u, int v)
 {
     const int s = t + u + v;
@@ -1171,13 +1201,14 @@ __device__ void mmd_accumulate(const MmdShell A, const MmdShell B, double Cx, do
     const int la = A.L, lb = B.L, N = la + lb, N1 = N + 1;
     const int ncA = dev_ncart(la), ncB = dev_ncart(lb), nel = ncA * ncB;
     const int esz = 3 * (la + 1) * (lb + 1) * N1;   // E for three directions
+    const int e0sz = 3 * (la + 1) * (lb + 1);        // their t = 0 columns (r^0 terms)
     const int ssz = (N + 1) * (N + 2) * (N + 3) / 6;  // simplex layer
     __shared__ double w_s[64], q_s[64];
     __shared__ int n_s;
     for (int tk = t0; tk < t1; tk++) {
         const int npow = tpow[tk];
         const double dk = tcoef[tk], zk = tzeta[tk];
-        const int per = (npow == 2) ? esz : esz + 2 * ssz + 64;
+        const int per = (npow == 2) ? e0sz : esz + 2 * ssz + 64;
         int bmax = scratch_len / per;
         if (bmax > 64) bmax = 64;
         if (npow != 2) bmax = 1;
@@ -1196,7 +1227,10 @@ __device__ void mmd_accumulate(const MmdShell A, const MmdShell B, double Cx, do
                 const double Qd = (a * Ad + b * Bd + c * Cd) / q;
                 const double XAB = Ad - Bd, XAC = Ad - Cd, XBC = Bd - Cd;
                 const double K = exp(-(a * b / q) * XAB * XAB) * exp(-(a * c / q) * XAC * XAC) * exp(-(b * c / q) * XBC * XBC);
-                mmd_E_dir(scratch + pb * per + dir * (la + 1) * (lb + 1) * N1, la, lb, q, K, Qd - Ad, Qd - Bd);
+                if (npow == 2)
+                    mmd_E0_dir(scratch + pb * per + dir * (la + 1) * (lb + 1), la, lb, q, K, Qd - Ad, Qd - Bd);
+                else
+                    mmd_E_dir(scratch + pb * per + dir * (la + 1) * (lb + 1) * N1, la, lb, q, K, Qd - Ad, Qd - Bd);
                 if (dir == 0) {
                     const double ca = craw[A.poff + ia], cb = craw[B.poff + ib];
                     const double skip = (fabs(ca) < 1e-14 || fabs(cb) < 1e-14) ? 0.0 : 1.0;
@@ -1290,8 +1324,8 @@ __device__ void mmd_accumulate(const MmdShell A, const MmdShell B, double Cx, do
                     int ax, ay, az, bx, by, bz;
                     dev_cart(la, i, ax, ay, az);
                     dev_cart(lb, j, bx, by, bz);
-                    const int ox = (ax * (lb + 1) + bx) * N1, oy = (la + 1) * (lb + 1) * N1 + (ay * (lb + 1) + by) * N1,
-                              oz = 2 * (la + 1) * (lb + 1) * N1 + (az * (lb + 1) + bz) * N1;
+                    const int ox = ax * (lb + 1) + bx, oy = (la + 1) * (lb + 1) + ay * (lb + 1) + by,
+                              oz = 2 * (la + 1) * (lb + 1) + az * (lb + 1) + bz;
                     double sacc = 0.0;
                     for (int pb = 0; pb < nb; pb++) {
                         const double *E = scratch + pb * per;
@@ -2342,14 +2376,14 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
                 ocvlen += 2LL * ncart(sh[A].L) * (2 * sh[oce[o].shell].L + 1);
                 ocv.push_back(it);
                 // LDS of k_ocv that batches the primitive pairs exactly as the 96 KiB bound would
-                // (mmd_accumulate takes min(64, scratch / esz) pairs per batch for r^0 terms, one
+                // (mmd_accumulate takes min(64, scratch / e0sz) pairs per batch for r^0 terms, one
                 // pair with two R/G layers otherwise), so the sums are the same
                 const int la = sh[A].L, lo = sh[oce[o].shell].L, N = la + lo;
-                const long long esz = 3LL * (la + 1) * (lo + 1) * (N + 1);
+                const long long esz = 3LL * (la + 1) * (lo + 1) * (N + 1), e0sz = 3LL * (la + 1) * (lo + 1);
                 const long long ssz = (long long) (N + 1) * (N + 2) * (N + 3) / 6;
                 const long long npair = (long long) sh[A].nprim * sh[oce[o].shell].nprim;
-                const long long bmax = std::min<long long>(64, MMD_SCRATCH / esz);
-                long long need = std::max<long long>(1, std::min(npair, bmax)) * esz;
+                const long long bmax = std::min<long long>(64, MMD_SCRATCH / e0sz);
+                long long need = std::max<long long>(1, std::min(npair, bmax)) * e0sz;
                 const DPot &op = pots[oce[o].pot];
                 for (int k = op.toff; k < op.toff + op.nterm; k++)
                     if (tpow[k] != 2) need = std::max(need, esz + 2 * ssz + 64);
@@ -2384,18 +2418,18 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
         }
     }
     {
-        // LDS of k_type1: E tables of a batch of primitive pairs (r^0 terms), or E plus two R/G
-        // layers of one pair (r^-1, r^-2 terms); the block accumulator
+        // LDS of k_type1: t = 0 E columns of a batch of primitive pairs (r^0 terms), or E plus two
+        // R/G layers of one pair (r^-1, r^-2 terms); the block accumulator
         bool other_pow = false;
         for (int v : tpow) other_pow |= (v != 2);
         long long sc = 64, acc = 1;
         for (int i = 0; i < nitem; i++) {
             const int la = sh[items[i].a].L, lb = sh[items[i].b].L, N = la + lb;
-            const long long esz = 3LL * (la + 1) * (lb + 1) * (N + 1);
+            const long long esz = 3LL * (la + 1) * (lb + 1) * (N + 1), e0sz = 3LL * (la + 1) * (lb + 1);
             const long long ssz = (long long) (N + 1) * (N + 2) * (N + 3) / 6;
             const long long npair = (long long) sh[items[i].a].nprim * sh[items[i].b].nprim;
-            sc = std::max(sc, std::min<long long>(MMD_SCRATCH_BATCH, npair * esz));
-            sc = std::max(sc, esz);
+            sc = std::max(sc, std::min<long long>(MMD_SCRATCH_BATCH, npair * e0sz));
+            sc = std::max(sc, e0sz);
             if (other_pow) sc = std::max(sc, esz + 2 * ssz + 64);
             acc = std::max<long long>(acc, (long long) ncart(la) * ncart(lb));
         }
@@ -2655,7 +2689,8 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
     const bool ser = P->serial;
     auto rec = [&](hipEvent_t e, hipStream_t s) { return ser ? hipSuccess : hipEventRecord(e, s); };
     auto wait = [&](hipStream_t s, hipEvent_t e) { return ser ? hipSuccess : hipStreamWaitEvent(s, e, 0); };
-    hipStream_t side = ser ? st : P->side;
+    // layout 2 runs every kernel on the caller's stream (nothing shares the chip)
+    hipStream_t side = (ser || P->tune.layout == 2) ? st : P->side;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, P->d_counters, 8);
     HIPCHK(rec(P->ev[0], st));
     HIPCHK(wait(side, P->ev[0]));
@@ -2901,9 +2936,9 @@ extern "C" int eng_plan_set_tuning(eng_plan_t *P, const eng_tuning_t *t)
 {
     auto in3 = [](int v, int a, int b, int c) { return v == a || v == b || v == c; };
     if (!in3(t->screen_probe, 1, 2, 4) || !in3(t->screen_split, 4, 8, 16) || !in3(t->contract_threads, 64, 128, 256) ||
-        !in3(t->type1_threads, 64, 128, 256) || (t->layout != 0 && t->layout != 1)) {
+        !in3(t->type1_threads, 64, 128, 256) || !in3(t->layout, 0, 1, 2)) {
         eng_set_error("tuning: screen_probe must be 1, 2 or 4, screen_split 4, 8 or 16, the thread counts 64, 128 or "
-                      "256, the layout 0 or 1");
+                      "256, the layout 0, 1 or 2");
         return 2;
     }
     P->tune = *t;

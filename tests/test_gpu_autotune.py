@@ -1,6 +1,6 @@
 """The autotuner changes launch shapes only: every shape it may pick gives bitwise the matrices of
 the default shapes (screening probe width 1/2/4 and 4/8/16 waves per tile, contraction and type-1
-workgroups of 64/128/256 threads, either stream layout), and a tuned plan, eager or graph-replayed, still does."""
+workgroups of 64/128/256 threads, any of the three stream layouts), and a tuned plan, eager or graph-replayed, still does."""
 import numpy as np
 import pytest
 
@@ -17,7 +17,8 @@ def test_every_launch_shape_gives_the_same_matrices(workload):
     plan.execute()
     want = plan.download()
     for shape in ((4, 64, 64, 1, 16), (2, 256, 128, 0, 8), (1, 128, 256, 1, 4), (4, 256, 64, 0, 16),
-                  (2, 64, 256, 1, 8), (1, 128, 128, 0, 16), (1, 128, 128, 0, 4), (1, 128, 128, 0, 0)):
+                  (2, 64, 256, 1, 8), (1, 128, 128, 0, 16), (1, 128, 128, 0, 4), (1, 128, 128, 0, 0),
+                  (1, 128, 256, 2, 4), (2, 64, 128, 2, 16)):
         plan.set_tuning(*shape)
         plan.execute()
         got = plan.download()
