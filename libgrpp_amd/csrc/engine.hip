@@ -680,7 +680,9 @@ __global__ __launch_bounds__(256) void k_canon(const TileItem *__restrict__ tile
         const int cn = a_first ? dev_canon_nA(L, la, lb, ra.lam, rb.lam, N) : dev_canon_nA_max(L, la, lb, ra.lam, rb.lam, N);
         const int r2 = ra.lidx - dev_kcol(L, ra.n, ra.lam) + dev_kcol(L, cn, ra.lam);
         const int c2 = rb.lidx - dev_kcol(L, rb.n, rb.lam) + dev_kcol(L, N - cn, rb.lam);
-        Qc[jb.qoff + (long long) row * jb.M + col] =
+        // written in the LOGICAL row order: the contraction reads a shell pair's block as KB
+        // consecutive doubles per row, with no row permutation to look up
+        Qc[jb.qoff + (long long) ra.lidx * jb.M + rb.lidx] =
             Q[jb.qoff + (long long) J.rowpos[jb.rows + r2] * jb.M + J.rowpos[jb.rows + c2]];
     }
 }
@@ -1574,7 +1576,7 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
                     const int rb = J.slot_rowoff[(long long) p.slot * J.nshell + it.b];
                     for (int e = tid; e < KA * KB; e += blockDim.x) {
                         const int k1 = e / KB, k2 = e % KB;
-                        Qs[e] = J.Q[p.qoff + (long long) J.rowpos[p.rbase + ra + k1] * p.M + J.rowpos[p.rbase + rb + k2]];
+                        Qs[e] = J.Q[p.qoff + (long long) (ra + k1) * p.M + rb + k2];   // logical order
                     }
                 }
                 __syncthreads();
@@ -3023,8 +3025,9 @@ extern "C" int eng_plan_set_predictions(eng_plan_t *P, int n, const float *pred)
     return 0;
 }
 
-// diagnostics: copies a radial buffer of the last execute (0 = Q, 1 = canonical Q, 2 = screening
-// estimates) and the row permutation (logical -> physical) to the host
+// diagnostics: copies a radial buffer of the last execute (0 = Q in the physical row order, 1 =
+// canonical Q and 2 = screening estimates in the logical order) and the row permutation (logical
+// -> physical) to the host
 extern "C" int eng_plan_debug_table(eng_plan_t *P, int which, double *out, int *rowpos, long long *qoff, int *M,
                                     int *rows)
 {
