@@ -120,6 +120,22 @@ long long libgrpp_batch_plan_output_len(const libgrpp_batch_plan_t *plan);   /* 
 int libgrpp_batch_plan_download_flat(libgrpp_batch_plan_t *plan, double *out);   /* 4 * output_len doubles */
 int libgrpp_batch_plan_dim(const libgrpp_batch_plan_t *plan);
 /*
+ * Spin-orbital GRPP Hamiltonian of a matrix plan's last execute (no counterpart in the reference,
+ * which leaves this step to the host: doc/manual.tex:594).  libgrpp's so_eta are real, with
+ * <A| U^SO P l_eta |B> = i so_eta[A][B], and the spin s = sigma/2 is not applied; this assembles
+ *   H = [ A + i Z/2        Y/2 + i X/2 ]     A = T arep T^T, (X, Y, Z) = T so_(x,y,z) T^T,
+ *       [ -Y/2 + i X/2     A - i Z/2   ]
+ * over spin-orbitals (all alpha functions, then all beta), a complex Hermitian matrix of order
+ * 2n, row-major with interleaved (re, im) doubles.  spherical = 0: T is the identity (n = number of
+ * Cartesian functions); spherical = 1: T applies, per shell, the Cartesian -> real spherical
+ * harmonic coefficients of libgrpp's outercore code (outercore_integrals.c:171-178), n = sum of
+ * 2l+1.  A molecules plan gives one such matrix per molecule, one after the other.
+ * _len: complex elements H must hold (-1 for a gradient plan).
+ */
+long long libgrpp_batch_plan_spin_hamiltonian_len(const libgrpp_batch_plan_t *plan, int spherical);
+int libgrpp_batch_plan_spin_hamiltonian(libgrpp_batch_plan_t *plan, int spherical, double *H);
+
+/*
  * Autotuner: launch shapes of a plan that change no result (bitwise), picked by timing the plan's
  * own build: the screening probe width and waves per tile (per-phase time of the screening), the contraction's and
  * the type-1 workgroup sizes (contraction phase, whole build) and the stream layout (whole build).  Each candidate runs `repeats`

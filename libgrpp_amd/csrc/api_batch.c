@@ -290,7 +290,32 @@ struct libgrpp_batch_plan {
     int flags;
     long long out_len;   /* doubles per operator in the engine's output */
     grad_host_t *grad;   /* gradient plans only */
+    /* matrix plans: shell angular momenta and the molecules' shell ranges and output offsets
+       (one molecule for a single-system plan), for the spin-orbital Hamiltonian */
+    int nmol;
+    int *shell_L;
+    int *mol_shell_off;
+    long long *mol_cart_off;
 };
+
+/* records the layout of a matrix plan's output (nmol molecules, shells [off[m], off[m+1])) */
+static void keep_layout(libgrpp_batch_plan_t *p, int nmol, const int *shell_off, libgrpp_shell_t *const *shells)
+{
+    const int ns = shell_off[nmol];
+    p->nmol = nmol;
+    p->shell_L = (int *) malloc(sizeof(int) * (ns > 0 ? ns : 1));
+    p->mol_shell_off = (int *) malloc(sizeof(int) * (nmol + 1));
+    p->mol_cart_off = (long long *) malloc(sizeof(long long) * (nmol > 0 ? nmol : 1));
+    long long off = 0;
+    for (int m = 0; m <= nmol; m++) p->mol_shell_off[m] = shell_off[m];
+    for (int i = 0; i < ns; i++) p->shell_L[i] = shells[i]->L;
+    for (int m = 0; m < nmol; m++) {
+        long long d = 0;
+        for (int i = shell_off[m]; i < shell_off[m + 1]; i++) d += shells[i]->cart_size;
+        p->mol_cart_off[m] = off;
+        off += d * d;
+    }
+}
 
 static int check_canonical(int n, libgrpp_shell_t *const *shells)
 {
@@ -395,6 +420,10 @@ static int batch_plan_create(int nshells, libgrpp_shell_t *const *shells, int nc
     p->dim = dim;
     p->flags = flags;
     p->out_len = n2;
+    {
+        const int so[2] = {0, nshells};
+        keep_layout(p, 1, so, shells);
+    }
     *plan = p;
     return 0;
 }
@@ -479,6 +508,7 @@ int libgrpp_batch_molecules_plan_create(int nmol, const int *shell_off, const in
     p->dim = 0;
     p->flags = flags;
     p->out_len = total;
+    keep_layout(p, nmol, shell_off, shells);
     *plan = p;
     return 0;
 }
@@ -596,7 +626,36 @@ void libgrpp_batch_plan_destroy(libgrpp_batch_plan_t *p)
     if (!p) return;
     eng_plan_destroy(p->eng);
     grad_host_free(p->grad);
+    free(p->shell_L);
+    free(p->mol_shell_off);
+    free(p->mol_cart_off);
     free(p);
+}
+
+long long libgrpp_batch_plan_spin_hamiltonian_len(const libgrpp_batch_plan_t *p, int spherical)
+{
+    if (p->grad || !p->shell_L) return -1;
+    long long n = 0;
+    for (int m = 0; m < p->nmol; m++) {
+        long long d = 0;
+        for (int i = p->mol_shell_off[m]; i < p->mol_shell_off[m + 1]; i++) {
+            const int l = p->shell_L[i];
+            d += spherical ? 2 * l + 1 : (l + 1) * (l + 2) / 2;
+        }
+        n += 4 * d * d;
+    }
+    return n;
+}
+
+int libgrpp_batch_plan_spin_hamiltonian(libgrpp_batch_plan_t *p, int spherical, double *H)
+{
+    if (p->grad || !p->shell_L) {
+        eng_set_error("spin_hamiltonian: a gradient plan has no matrices to assemble");
+        return 2;
+    }
+    long long len = 0;
+    return eng_plan_spin_hamiltonian(p->eng, p->shell_L, p->mol_shell_off, p->nmol, p->mol_cart_off, spherical != 0,
+                                     H, &len);
 }
 
 int libgrpp_batch_full_grpp_integrals(int nshells, libgrpp_shell_t *const *shells, int ncenters, const double *xyz,

@@ -68,6 +68,10 @@ int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int ncenter, con
                     long long op_stride, int parts, int flags, eng_plan_t **plan);
 int eng_plan_execute(eng_plan_t *plan, void *stream);
 int eng_plan_download(eng_plan_t *plan, double *out);   /* out_len * 4 doubles */
+/* spin-orbital one-electron Hamiltonian of the last execute (see engine.hip, k_spin_ham); with
+   H_host NULL only *h_len (complex elements) is set */
+int eng_plan_spin_hamiltonian(eng_plan_t *plan, const int *shell_L, const int *mol_shell_off, int nmol,
+                              const long long *mol_cart_off, int spherical, double *H_host, long long *h_len);
 /* the four operators' first out_len doubles straight into four host arrays */
 int eng_plan_download_ops(eng_plan_t *plan, double *o0, double *o1, double *o2, double *o3);
 double *eng_plan_device_output(eng_plan_t *plan);

@@ -2856,6 +2856,132 @@ static int eager_phase_times(eng_plan_t *P)
     return 0;
 }
 
+// ================================================================== spin-orbital Hamiltonian
+// The step after the integrals (doc/manual.tex:594): libgrpp returns the real matrices arep and
+// so_eta, where <A| U^SO P l_eta |B> = i so_eta[A][B] (its L matrices hold Im <S_m|l_eta|S_m'>),
+// and leaves the spin s = sigma/2 to the host.  The one-electron GRPP Hamiltonian in the basis of
+// spin-orbitals (all alpha functions, then all beta) is the complex Hermitian matrix
+//   H = [ A + i Z/2        Y/2 + i X/2 ]
+//       [ -Y/2 + i X/2     A - i Z/2   ],   A = T arep T^T, (X, Y, Z) = T so_(x,y,z) T^T,
+// with T the identity (Cartesian functions) or, per shell, the Cartesian -> real-spherical-harmonic
+// coefficients of libgrpp's outercore code (outercore_integrals.c:171-178, applied on both sides as
+// in transform_to_sph_basis_ket/_bra, :505-542).  One workgroup per ordered shell pair.
+struct HamItem {
+    int la, lb, sph_row, sph_col, nsph;   // block position in the molecule's spherical (or Cartesian) dim
+    int cart_stride;                      // row stride of the Cartesian matrices
+    long long cart_off, h_off;            // first Cartesian element of the block; first element of H
+};
+
+constexpr int HAM_NC = (LMAX_SHELL + 1) * (LMAX_SHELL + 2) / 2, HAM_NS = 2 * LMAX_SHELL + 1;
+
+__global__ __launch_bounds__(64) void k_spin_ham(const HamItem *__restrict__ items, const double *__restrict__ out4,
+                                                 long long op_stride, DevTables T, int spherical, double2 *__restrict__ H)
+{
+    __shared__ double Y[HAM_NC * HAM_NS];
+    __shared__ double Z[4][HAM_NS * HAM_NS];
+    const HamItem it = items[blockIdx.x];
+    const int ncA = dev_ncart(it.la), ncB = dev_ncart(it.lb);
+    const int nsA = spherical ? 2 * it.la + 1 : ncA, nsB = spherical ? 2 * it.lb + 1 : ncB;
+    const double *TA = T.slm + T.slm_off[it.la], *TB = T.slm + T.slm_off[it.lb];
+    for (int op = 0; spherical && op < 4; op++) {
+        const double *X = out4 + (long long) op * op_stride + it.cart_off;
+        // Y = X T_B^T, then Z = T_A Y
+        for (int e = threadIdx.x; e < ncA * nsB; e += blockDim.x) {
+            const int a = e / nsB, q = e % nsB;
+            double acc = 0.0;
+            for (int b = 0; b < ncB; b++) acc += X[(long long) a * it.cart_stride + b] * TB[q * ncB + b];
+            Y[e] = acc;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < nsA * nsB; e += blockDim.x) {
+            const int pp = e / nsB, q = e % nsB;
+            double acc = 0.0;
+            for (int a = 0; a < ncA; a++) acc += TA[pp * ncA + a] * Y[a * nsB + q];
+            Z[op][e] = acc;
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    const long long n = it.nsph, n2 = 2LL * it.nsph;
+    for (int e = threadIdx.x; e < nsA * nsB; e += blockDim.x) {
+        const long long r = it.sph_row + e / nsB, c = it.sph_col + e % nsB;
+        double v[4];
+        for (int op = 0; op < 4; op++)
+            v[op] = spherical ? Z[op][e]
+                              : out4[(long long) op * op_stride + it.cart_off + (long long) (e / nsB) * it.cart_stride + e % nsB];
+        const double a = v[0], x = 0.5 * v[1], y = 0.5 * v[2], z = 0.5 * v[3];
+        H[it.h_off + r * n2 + c] = make_double2(a, z);
+        H[it.h_off + (n + r) * n2 + n + c] = make_double2(a, -z);
+        H[it.h_off + r * n2 + n + c] = make_double2(y, x);
+        H[it.h_off + (n + r) * n2 + c] = make_double2(-y, x);
+    }
+}
+
+// molecules m = 0..nmol-1 own shells [mol_shell_off[m], mol_shell_off[m+1]) of shell_L and their
+// Cartesian matrices start at mol_cart_off[m] of each operator; H (host, interleaved complex)
+// receives, molecule after molecule, the (2 n_m) x (2 n_m) spin-orbital matrices
+extern "C" int eng_plan_spin_hamiltonian(eng_plan_t *P, const int *shell_L, const int *mol_shell_off, int nmol,
+                                         const long long *mol_cart_off, int spherical, double *H_host,
+                                         long long *h_len)
+{
+    std::vector<HamItem> items;
+    long long hlen = 0;
+    for (int m = 0; m < nmol; m++) {
+        const int s0 = mol_shell_off[m], s1 = mol_shell_off[m + 1];
+        int ncart_m = 0, nsph_m = 0;
+        std::vector<int> coff, soff;
+        for (int a = s0; a < s1; a++) {
+            if (shell_L[a] < 0 || shell_L[a] > LMAX_SHELL) {
+                eng_set_error("spin_hamiltonian: shell %d has l = %d outside 0..%d", a, shell_L[a], LMAX_SHELL);
+                return 2;
+            }
+            coff.push_back(ncart_m);
+            soff.push_back(nsph_m);
+            ncart_m += ncart(shell_L[a]);
+            nsph_m += spherical ? 2 * shell_L[a] + 1 : ncart(shell_L[a]);
+        }
+        for (int a = s0; a < s1; a++)
+            for (int b = s0; b < s1; b++) {
+                HamItem it;
+                it.la = shell_L[a];
+                it.lb = shell_L[b];
+                it.sph_row = soff[a - s0];
+                it.sph_col = soff[b - s0];
+                it.nsph = nsph_m;
+                it.cart_stride = ncart_m;
+                it.cart_off = mol_cart_off[m] + (long long) coff[a - s0] * ncart_m + coff[b - s0];
+                it.h_off = hlen;
+                items.push_back(it);
+            }
+        hlen += 4LL * nsph_m * nsph_m;
+    }
+    *h_len = hlen;
+    if (!H_host) return 0;
+    if (items.empty()) return 0;
+    HamItem *d_items = nullptr;
+    double2 *d_H = nullptr;
+    HIPCHK(hipMalloc((void **) &d_items, items.size() * sizeof(HamItem)));
+    if (hipMalloc((void **) &d_H, (size_t) hlen * sizeof(double2)) != hipSuccess) {
+        (void) hipFree(d_items);
+        eng_set_error("spin_hamiltonian: %lld complex elements do not fit on the device", hlen);
+        return 1;
+    }
+    hipError_t e = hipMemcpy(d_items, items.data(), items.size() * sizeof(HamItem), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_spin_ham, dim3((unsigned) items.size()), dim3(64), 0, 0, d_items, P->d_out, P->op_stride,
+                           P->T, spherical, d_H);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(H_host, d_H, (size_t) hlen * sizeof(double2), hipMemcpyDeviceToHost);
+    (void) hipFree(d_items);
+    (void) hipFree(d_H);
+    if (e != hipSuccess) {
+        eng_set_error("spin_hamiltonian: %s", hipGetErrorString(e));
+        return 1;
+    }
+    return 0;
+}
+
 extern "C" int eng_plan_download(eng_plan_t *P, double *out)
 {
     HIPCHK(hipMemcpy(out, P->d_out, (size_t) 4 * P->op_stride * sizeof(double), hipMemcpyDeviceToHost));

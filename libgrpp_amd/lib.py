@@ -220,6 +220,25 @@ class _LevelHooks:
                                                         pred.ctypes.data_as(C.POINTER(C.c_float))),
                "libgrpp_batch_plan_set_predictions")
 
+def _spin_hamiltonian(h, systems, spherical: bool):
+    """libgrpp_batch_plan_spin_hamiltonian of a matrix plan: one complex (2n x 2n) matrix per molecule"""
+    L = lib()
+    L.libgrpp_batch_plan_spin_hamiltonian_len.restype = C.c_longlong
+    L.libgrpp_batch_plan_spin_hamiltonian_len.argtypes = [C.c_void_p, C.c_int]
+    L.libgrpp_batch_plan_spin_hamiltonian.restype = C.c_int
+    L.libgrpp_batch_plan_spin_hamiltonian.argtypes = [C.c_void_p, C.c_int, dptr]
+    n = L.libgrpp_batch_plan_spin_hamiltonian_len(h, int(spherical))
+    flat = np.zeros(2 * max(n, 1))
+    _check(L.libgrpp_batch_plan_spin_hamiltonian(h, int(spherical), as_dptr(flat)), "libgrpp_batch_plan_spin_hamiltonian")
+    z = flat[0::2] + 1j * flat[1::2]
+    out, o = [], 0
+    for m in systems:
+        d = sum((2 * s.L + 1) if spherical else s.cart_size for s in m.shells)
+        out.append(z[o:o + 4 * d * d].reshape(2 * d, 2 * d))
+        o += 4 * d * d
+    return out
+
+
 class Plan(_LevelHooks):
     """Packed, device-resident batched build: execute() runs the kernels only."""
 
@@ -239,6 +258,10 @@ class Plan(_LevelHooks):
         out = [np.zeros((self.dim, self.dim)) for _ in range(4)]
         _check(lib().libgrpp_batch_plan_download(self.h, *[as_dptr(m) for m in out]), "libgrpp_batch_plan_download")
         return out
+
+    def spin_hamiltonian(self, spherical: bool = True):
+        """complex spin-orbital Hamiltonian (2n x 2n, alpha functions first) of the last execute"""
+        return _spin_hamiltonian(self.h, [self.sysm], spherical)[0]
 
     def stats(self):
         st = Stats()
@@ -287,6 +310,10 @@ class MoleculesPlan(_LevelHooks):
             out.append([flat[k, o:o + d * d].reshape(d, d).copy() for k in range(4)])
             o += d * d
         return out
+
+    def spin_hamiltonian(self, spherical: bool = True):
+        """one complex spin-orbital Hamiltonian per molecule (see Plan.spin_hamiltonian)"""
+        return _spin_hamiltonian(self.h, self.systems, spherical)
 
     def stats(self):
         st = Stats()
