@@ -162,6 +162,59 @@ void libgrpp_full_grpp_integrals_gradient(libgrpp_shell_t *shell_A, libgrpp_shel
 /* overlap of two contracted shells (overlap.h / overlap.c:39), used by hosts for normalisation */
 void libgrpp_overlap_integrals(libgrpp_shell_t *shell_A, libgrpp_shell_t *shell_B, double *overlap_matrix);
 
+/*
+ * Nuclear attraction of a shell pair for a nucleus of charge `charge` at charge_origin
+ * (libgrpp.h:244-316, nuclear_attraction.c:54-256).  Models: the point charge, analytic
+ * (type-1 McMurchie-Davidson with an r^-1 term); a charged ball (model_params[0] = R_rms), a
+ * Gaussian distribution (R_rms), a Fermi distribution (c, a), Fermi with a quadratic "bubble" term
+ * (c, a, k) and the point charge again by quadrature: numeric radial type-1 integrals on the Log3
+ * grid (radial_type1_integral.c), shells with l <= 6.  Lengths in bohr.  Output as the other
+ * integrators (overwritten, row-major).
+ */
+enum {
+    LIBGRPP_NUCLEAR_MODEL_POINT_CHARGE = 0,
+    LIBGRPP_NUCLEAR_MODEL_CHARGED_BALL,
+    LIBGRPP_NUCLEAR_MODEL_GAUSSIAN,
+    LIBGRPP_NUCLEAR_MODEL_FERMI,
+    LIBGRPP_NUCLEAR_MODEL_FERMI_BUBBLE,
+    LIBGRPP_NUCLEAR_MODEL_POINT_CHARGE_NUMERICAL
+};
+void libgrpp_nuclear_attraction_integrals(libgrpp_shell_t *shell_A, libgrpp_shell_t *shell_B, double *charge_origin,
+                                          int charge, int nuclear_model, double *model_params,
+                                          double *coulomb_matrix);                        /* libgrpp.h:256 */
+void libgrpp_nuclear_attraction_integrals_point_charge(libgrpp_shell_t *shell_A, libgrpp_shell_t *shell_B,
+                                                       double *charge_origin, int charge,
+                                                       double *coulomb_matrix);           /* libgrpp.h:267 */
+void libgrpp_nuclear_attraction_integrals_charged_ball(libgrpp_shell_t *shell_A, libgrpp_shell_t *shell_B,
+                                                       double *charge_origin, int charge, double r_rms,
+                                                       double *coulomb_matrix);           /* libgrpp.h:276 */
+void libgrpp_nuclear_attraction_integrals_gaussian_model(libgrpp_shell_t *shell_A, libgrpp_shell_t *shell_B,
+                                                         double *charge_origin, int charge, double r_rms,
+                                                         double *coulomb_matrix);         /* libgrpp.h:286 */
+void libgrpp_nuclear_attraction_integrals_fermi_model(libgrpp_shell_t *shell_A, libgrpp_shell_t *shell_B,
+                                                      double *charge_origin, int charge, double fermi_param_c,
+                                                      double fermi_param_a, double *coulomb_matrix); /* libgrpp.h:296 */
+void libgrpp_nuclear_attraction_integrals_fermi_bubble_model(libgrpp_shell_t *shell_A, libgrpp_shell_t *shell_B,
+                                                             double *charge_origin, int charge, double param_c,
+                                                             double param_a, double param_k,
+                                                             double *coulomb_matrix);     /* libgrpp.h:307 */
+
+/* nuclear charge distribution models (nuclear_models.h:13-39): radii in fm for the estimates */
+double libgrpp_estimate_nuclear_rms_radius_johnson_1985(int A);
+double libgrpp_estimate_nuclear_rms_radius_golovko_2008(int A);
+int libgrpp_estimate_fermi_model_parameters(double R_rms, double *c, double *a);
+double libgrpp_charge_density_ball(double r, double Z, double R_rms);
+double libgrpp_charge_density_gaussian(double r, double Z, double R_rms);
+double libgrpp_charge_density_fermi(double r, double Z, double c, double a);
+double libgrpp_charge_density_fermi_bubble(double r, double Z, double c, double a, double k);
+double libgrpp_rms_radius_fermi(int Z, double c, double a);
+double libgrpp_rms_radius_fermi_bubble(int Z, double c, double a, double k);
+double libgrpp_coulomb_potential_point(double r, double Z);
+double libgrpp_coulomb_potential_ball(double r, double Z, double R_rms);
+double libgrpp_coulomb_potential_gaussian(double r, double Z, double R_rms);
+double libgrpp_coulomb_potential_fermi(double r, double Z, double c, double a);
+double libgrpp_coulomb_potential_fermi_bubble(double r, double Z, double c, double a, double k);
+
 #ifdef __cplusplus
 }
 #endif

@@ -86,6 +86,18 @@ int libgrpp_batch_full_grpp_integrals_noalloc(int nshells, libgrpp_shell_t *cons
                                               void *stream, libgrpp_batch_stats_t *stats);
 
 /*
+ * Nuclear attraction matrix of a whole shell list (no counterpart in the reference, whose
+ * nuclear-attraction call is per shell pair): V = sum over the nnuc nuclei of the attraction blocks
+ * of every ordered shell pair, dim x dim row-major (dim = sum of cart_size).  nuclear_model as for
+ * libgrpp_nuclear_attraction_integrals, the same for every nucleus; model_params holds 3 doubles per
+ * nucleus (R_rms, or c, a[, k]).  The point-charge model runs the analytic type-1 kernels, the
+ * others the numeric radial quadrature (l <= 6).
+ */
+int libgrpp_batch_nuclear_attraction_integrals(int nshells, libgrpp_shell_t *const *shells, int nnuc,
+                                               const double *nuc_xyz, const int *charges, int nuclear_model,
+                                               const double *model_params, double *V);
+
+/*
  * Nuclear gradients of the same matrices with respect to every centre's coordinates, treating
  * the centre list as the atom list (eval_integrals_grad.c:37-163 with the reference's
  * libgrpp_full_grpp_integrals_gradient per task).  out holds 4 * 3*ncenters * dim * dim doubles:

@@ -51,6 +51,9 @@ def build(verbose: bool = False) -> str:
         (["gcc", "-O3", "-fPIC", "-std=gnu11", "-Wall"] + inc + ["-c", os.path.join(CSRC, "fortran.c"), "-o",
                                                                  os.path.join(BUILD, "fortran.o")],
          os.path.join(BUILD, "fortran.o")),
+        (["gcc", "-O3", "-fPIC", "-std=gnu11", "-Wall"] + inc + ["-c", os.path.join(CSRC, "nuclear_models.c"), "-o",
+                                                                 os.path.join(BUILD, "nuclear_models.o")],
+         os.path.join(BUILD, "nuclear_models.o")),
     ]
     for cmd, obj in steps:
         out = _run(cmd)
@@ -58,7 +61,7 @@ def build(verbose: bool = False) -> str:
             print(out)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "-o", tmp] + objs + ["-lm"])
+    _run([HIPCC, "-shared", "-fPIC", "-o", tmp] + objs + ["-lm", "-lpthread"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -337,6 +337,85 @@ void libgrpp_type1_integrals(libgrpp_shell_t *A, libgrpp_shell_t *B, double *C, 
     free(buf);
 }
 
+/* ---------------------------------------------------------------- nuclear attraction */
+/* nuclear_attraction.c:54-133: the point charge through the type-1 kernels (a -Z r^-1 term of
+ * exponent 0), the finite models (and the numeric point charge) through the numeric radial
+ * quadrature on the device */
+void libgrpp_nuclear_attraction_integrals(libgrpp_shell_t *A, libgrpp_shell_t *B, double *C, int charge, int model,
+                                          double *params, double *matrix)
+{
+    require_init("libgrpp_nuclear_attraction_integrals");
+    const int sz = A->cart_size * B->cart_size;
+    memset(matrix, 0, sizeof(double) * sz);
+    if (model == LIBGRPP_NUCLEAR_MODEL_POINT_CHARGE) {
+        int pw = 1;
+        double cf = -(double) charge, ze = 0.0;
+        libgrpp_potential_t *u = libgrpp_new_potential(0, 0, 1, &pw, &cf, &ze);
+        eng_center_t c;
+        memset(&c, 0, sizeof(c));
+        c.origin = C;
+        c.local = u;
+        c.shell_end = -1;
+        double *buf = (double *) malloc(sizeof(double) * 4 * sz);
+        run_task("libgrpp_nuclear_attraction_integrals", A, B, &c, ENG_TYPE1, buf);
+        memcpy(matrix, buf, sizeof(double) * sz);
+        free(buf);
+        libgrpp_delete_potential(u);
+        return;
+    }
+    if (model < LIBGRPP_NUCLEAR_MODEL_CHARGED_BALL || model > LIBGRPP_NUCLEAR_MODEL_POINT_CHARGE_NUMERICAL) {
+        eng_set_error("libgrpp_nuclear_attraction_integrals: unknown nuclear model %d", model);
+        fail("libgrpp_nuclear_attraction_integrals");
+    }
+    libgrpp_shell_t *sh[2] = {A, B};
+    const int nAf = (A->L + 1) * (A->L + 2) / 2, nBf = (B->L + 1) * (B->L + 2) / 2;
+    eng_item_t it = {0, 1, 0, 1, 0, 0, nBf, 0};
+    double p3[3] = {0.0, 0.0, 0.0};
+    const int np = model == LIBGRPP_NUCLEAR_MODEL_CHARGED_BALL || model == LIBGRPP_NUCLEAR_MODEL_GAUSSIAN ? 1
+                   : model == LIBGRPP_NUCLEAR_MODEL_FERMI ? 2 : model == LIBGRPP_NUCLEAR_MODEL_FERMI_BUBBLE ? 3 : 0;
+    for (int i = 0; i < np; i++) p3[i] = params[i];
+    double *buf = (double *) malloc(sizeof(double) * nAf * nBf);
+    if (eng_nuclear(2, sh, 1, C, &charge, model, p3, libgrpp_params.radial_tolerance, 1, &it, (long long) nAf * nBf,
+                    buf)) {
+        free(buf);
+        fail("libgrpp_nuclear_attraction_integrals");
+    }
+    to_shell_order(A, B, buf, matrix);
+    free(buf);
+}
+
+void libgrpp_nuclear_attraction_integrals_point_charge(libgrpp_shell_t *A, libgrpp_shell_t *B, double *C, int charge,
+                                                       double *matrix)
+{
+    libgrpp_nuclear_attraction_integrals(A, B, C, charge, LIBGRPP_NUCLEAR_MODEL_POINT_CHARGE, NULL, matrix);
+}
+
+void libgrpp_nuclear_attraction_integrals_charged_ball(libgrpp_shell_t *A, libgrpp_shell_t *B, double *C, int charge,
+                                                       double r_rms, double *matrix)
+{
+    libgrpp_nuclear_attraction_integrals(A, B, C, charge, LIBGRPP_NUCLEAR_MODEL_CHARGED_BALL, &r_rms, matrix);
+}
+
+void libgrpp_nuclear_attraction_integrals_gaussian_model(libgrpp_shell_t *A, libgrpp_shell_t *B, double *C, int charge,
+                                                         double r_rms, double *matrix)
+{
+    libgrpp_nuclear_attraction_integrals(A, B, C, charge, LIBGRPP_NUCLEAR_MODEL_GAUSSIAN, &r_rms, matrix);
+}
+
+void libgrpp_nuclear_attraction_integrals_fermi_model(libgrpp_shell_t *A, libgrpp_shell_t *B, double *C, int charge,
+                                                      double c, double a, double *matrix)
+{
+    double p[2] = {c, a};
+    libgrpp_nuclear_attraction_integrals(A, B, C, charge, LIBGRPP_NUCLEAR_MODEL_FERMI, p, matrix);
+}
+
+void libgrpp_nuclear_attraction_integrals_fermi_bubble_model(libgrpp_shell_t *A, libgrpp_shell_t *B, double *C,
+                                                             int charge, double c, double a, double k, double *matrix)
+{
+    double p[3] = {c, a, k};
+    libgrpp_nuclear_attraction_integrals(A, B, C, charge, LIBGRPP_NUCLEAR_MODEL_FERMI_BUBBLE, p, matrix);
+}
+
 void libgrpp_type2_integrals(libgrpp_shell_t *A, libgrpp_shell_t *B, double *C, libgrpp_potential_t *pot,
                              double *matrix)
 {

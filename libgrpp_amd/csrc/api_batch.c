@@ -698,6 +698,81 @@ int libgrpp_batch_full_grpp_integrals_noalloc(int nshells, libgrpp_shell_t *cons
 }
 
 /*
+ * Nuclear attraction matrix of a shell list: every shell pair i <= j once, mirrored (the operator is
+ * real symmetric), summed over the nuclei.  The point-charge model runs the analytic type-1 kernels
+ * with one -Z r^-1 term of exponent 0 per nucleus (nuclear_attraction.c:73-84); the finite models
+ * and the numeric point charge run the numeric radial quadrature (nuclear_attraction.c:85-160).
+ */
+int libgrpp_batch_nuclear_attraction_integrals(int nshells, libgrpp_shell_t *const *shells, int nnuc,
+                                               const double *nuc_xyz, const int *charges, int nuclear_model,
+                                               const double *model_params, double *V)
+{
+    if (!check_canonical(nshells, shells)) return 2;
+    if (nuclear_model < LIBGRPP_NUCLEAR_MODEL_POINT_CHARGE || nuclear_model > LIBGRPP_NUCLEAR_MODEL_POINT_CHARGE_NUMERICAL) {
+        eng_set_error("batch_nuclear_attraction_integrals: unknown nuclear model %d", nuclear_model);
+        return 2;
+    }
+    if (nuclear_model != LIBGRPP_NUCLEAR_MODEL_POINT_CHARGE && nuclear_model != LIBGRPP_NUCLEAR_MODEL_POINT_CHARGE_NUMERICAL &&
+        model_params == NULL) {
+        eng_set_error("batch_nuclear_attraction_integrals: model %d needs model_params", nuclear_model);
+        return 2;
+    }
+    int *off = (int *) malloc(sizeof(int) * (nshells + 1));
+    off[0] = 0;
+    for (int i = 0; i < nshells; i++) off[i + 1] = off[i] + shells[i]->cart_size;
+    const int dim = off[nshells];
+    const int nit = nshells * (nshells + 1) / 2;
+    eng_item_t *it = (eng_item_t *) malloc(sizeof(eng_item_t) * (nit > 0 ? nit : 1));
+    int k = 0;
+    for (int i = 0; i < nshells; i++)
+        for (int j = i; j < nshells; j++) {
+            eng_item_t e;
+            e.a = i;
+            e.b = j;
+            e.cb = 0;
+            e.ce = nnuc;
+            e.off = (long long) off[i] * dim + off[j];
+            e.moff = (long long) off[j] * dim + off[i];
+            e.stride = dim;
+            e.mirror = i != j;
+            it[k++] = e;
+        }
+    free(off);
+    const long long n2 = (long long) dim * dim;
+    int rc;
+    if (nuclear_model == LIBGRPP_NUCLEAR_MODEL_POINT_CHARGE) {
+        libgrpp_potential_t **pot = (libgrpp_potential_t **) calloc(nnuc > 0 ? nnuc : 1, sizeof(*pot));
+        eng_center_t *cs = (eng_center_t *) calloc(nnuc > 0 ? nnuc : 1, sizeof(eng_center_t));
+        int *cidx = (int *) malloc(sizeof(int) * (nnuc > 0 ? nnuc : 1));
+        for (int n = 0; n < nnuc; n++) {
+            int pw = 1;
+            double cf = -(double) charges[n], ze = 0.0;
+            pot[n] = libgrpp_new_potential(0, 0, 1, &pw, &cf, &ze);
+            cs[n].origin = nuc_xyz + 3 * n;
+            cs[n].local = pot[n];
+            cs[n].shell_end = -1;
+            cidx[n] = n;
+        }
+        double *buf = (double *) malloc(sizeof(double) * 4 * (n2 > 0 ? n2 : 1));
+        rc = eng_run(nshells, shells, nnuc, cs, nnuc, cidx, nit, it, n2, n2, ENG_TYPE1, buf);
+        if (!rc) memcpy(V, buf, sizeof(double) * n2);
+        free(buf);
+        for (int n = 0; n < nnuc; n++) libgrpp_delete_potential(pot[n]);
+        free(pot);
+        free(cs);
+        free(cidx);
+    } else {
+        double *p3 = (double *) calloc(3 * (nnuc > 0 ? nnuc : 1), sizeof(double));
+        for (int i = 0; model_params && i < 3 * nnuc; i++) p3[i] = model_params[i];
+        rc = eng_nuclear(nshells, shells, nnuc, nuc_xyz, charges, nuclear_model, p3, libgrpp_params.radial_tolerance,
+                         nit, it, n2, V);
+        free(p3);
+    }
+    free(it);
+    return rc;
+}
+
+/*
  * Batched gradients: the engine evaluates, per centre, the blocks <i-|U_C|j> and <i+|U_C|j> for
  * every ordered pair; ket derivatives reuse them through <i|U|j+-> = s <j+-|U|i>^T
  * (s = +1 arep, -1 spin-orbit).  A gradient plan holds the engine plan of those blocks and the

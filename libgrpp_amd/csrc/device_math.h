@@ -20,6 +20,7 @@ struct DevTables {
     const int *wdir;          // [(L*(nmax+1)+n)] -> offset, -1 if absent
     const int *wstride;       // mu stride per (L, n)
     int w_nmax;
+    const double *w1;         // type-1 angular couplings of the numeric nuclear-attraction path
 };
 
 // K_n(x) = exp(-x) i_n(x): series head for tiny x, 4th-order Taylor about the nearest 0.01

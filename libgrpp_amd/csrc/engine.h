@@ -91,7 +91,13 @@ int eng_run(int nshell, libgrpp_shell_t *const *shells, int ncenter, const eng_c
 
 void eng_set_error(const char *fmt, ...);
 int eng_device_ok(void);
-int eng_setup_device(void);   /* tables, LDS limits, scratch address on the current device (if any) */
+int eng_setup_device(void);
+/* nuclear attraction by the numeric radial quadrature (models 1..5, l <= 6), summed over nuclei */
+int eng_nuclear(int nshell, libgrpp_shell_t *const *shells, int nnuc, const double *nuc_xyz, const int *charge,
+                int model, const double *params /* [nnuc][3] */, double tol, int nitem, const eng_item_t *items,
+                long long out_len, double *out);
+/* host: a nuclear model's potential at n radii (nuclear_models.c) */
+int nuc_model_potential(int model, double Z, const double *p, int n, const double *r, double *U);   /* tables, LDS limits, scratch address on the current device (if any) */
 
 #ifdef __cplusplus
 }

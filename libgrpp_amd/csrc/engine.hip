@@ -1794,11 +1794,14 @@ static contract_fn contract_kernel(int threads);
 // device tables are first set up (under g_tab_mutex), to everything a gfx950 CU offers beside the
 // kernel's static LDS; each launch still asks only for its plan's size, which alone sets occupancy.
 // Setting it per plan instead would let a thread lower it under another thread's larger launch.
+static const void *nuc_kernel(void);
+
 static int raise_lds_limits(void)
 {
     const int cu_lds = 160 * 1024;
     const void *fns[] = {(const void *) k_ocv, (const void *) k_type1, (const void *) contract_kernel(64),
-                         (const void *) contract_kernel(128), (const void *) contract_kernel(256)};
+                         (const void *) contract_kernel(128), (const void *) contract_kernel(256),
+                         nuc_kernel()};
     for (const void *f : fns) {
         hipFuncAttributes a{};
         HIPCHK(hipFuncGetAttributes(&a, f));
@@ -1887,6 +1890,8 @@ static int device_tables(int Lmax, int nmax, DevTables *out, bool no_alloc = fal
         D.t.slm = p;
         if (upload(H.slm_off, &ip, D.bufs)) return 1;
         D.t.slm_off = ip;
+        if (upload(H.w1, &p, D.bufs)) return 1;
+        D.t.w1 = p;
         D.ok = true;
     }
     if (Lmax > D.Lmax || nmax > D.nmax) {
@@ -2853,6 +2858,276 @@ static int eager_phase_times(eng_plan_t *P)
     HIPCHK(hipEventElapsedTime(&ms, P->ev[0], P->ev[7]));
     P->last_seconds = ms * 1e-3;
     P->last_radial_seconds = P->phase_seconds[2];
+    return 0;
+}
+
+// ================================================================== nuclear attraction, numeric radial type-1
+// Nuclear attraction of a finite (or point) nuclear charge distribution for one (shell A, shell B,
+// nucleus) item and one primitive pair per workgroup (nuclear_attraction.c:54-133 with
+// type1_integrals.c:144-270 and radial_type1_integral.c:100-325):
+//   radial table Q[lam][N] = int w U(r) D exp(k r - (aA + aB) r^2 - aA CA^2 - aB CB^2) r^N K_lam(k r) dr,
+//   lam, N <= la + lb, D = 4 pi N_A N_B, k = |k|, k = -2 (aA CA + aB CB),
+// on the Log3 grid from 2047 points (levels 0..6 of the level-ordered grid, each point's weight
+// scaled by the exact power of two 2^(level - 6)), doubled to 4095 and 8191: the entry keeps S7
+// when |S7 - S6| <= tol, else S8 (the reference's loop stops at 10000 points).  U is the model's
+// potential on the grid, tabulated on the host with the C library (as the reference evaluates it).
+// Then chi_ij = sum over the binomial expansion about C of factor * sum_lam Q[lam][N] Omega_lam,
+// Omega_lam(I, J, K) = sum_mu S_lam,mu(k) W1[lam][mu][(I, J, K)] (angular_integrals.c:35-56),
+// with the reference's skips (|factor| < 1e-13, |Q| < 1e-16); the workgroup writes
+// cA cB chi into its slice and k_nuc_sum adds the slices in primitive-pair order.
+struct NucWork {
+    int item, nucleus, ia, ib;   // (shell pair, nucleus, primitive pair)
+    long long slice;             // first double of its ncart(A) x ncart(B) slice
+};
+struct NucItem {
+    int a, b, w0, w1;            // shells; its work range [w0, w1) (nucleus-major, then primitive pairs)
+    long long off, moff;         // output block start (row stride `stride`); start of the mirrored (B, A) block
+    int stride, mirror;          // mirror: also write the transpose at moff
+};
+
+// x^n by repeated multiplication from 1 (the powers of spherical_harmonics.c:197-205)
+__device__ __forceinline__ double rep_pow(double x, int n)
+{
+    double r = 1.0;
+    for (int i = 0; i < n; i++) r *= x;
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_nuc_prim(const NucWork *__restrict__ work, const NucItem *__restrict__ items,
+                                                  DevJob J, DevTables T, const double *__restrict__ nuc_xyz,
+                                                  const double *__restrict__ U, double tol, double *__restrict__ slices)
+{
+    extern __shared__ double nuc_lds[];
+    const NucWork wk = work[blockIdx.x];
+    const NucItem it = items[wk.item];
+    const DShell A = J.sh[it.a], B = J.sh[it.b];
+    const int la = A.L, lb = B.L, lab = la + lb, L1 = lab + 1, ncA = dev_ncart(la), ncB = dev_ncart(lb);
+    const int nijk = L1 * (L1 + 1) * (L1 + 2) / 6;
+    double *Kc = nuc_lds, *Rc = Kc + L1 * 64, *Gc = Rc + L1 * 64, *Q = Gc + 64, *SY = Q + L1 * L1, *Om = SY + L1 * L1;
+    const int tid = threadIdx.x;
+    const double cA = J.craw[A.poff + wk.ia], cB = J.craw[B.poff + wk.ib];
+    double *out = slices + wk.slice;
+    if (fabs(cA * cB) < 1e-13) {   // nuclear_attraction.c:66
+        for (int e = tid; e < ncA * ncB; e += blockDim.x) out[e] = 0.0;
+        return;
+    }
+    const double aA = J.alpha[A.poff + wk.ia], aB = J.alpha[B.poff + wk.ib];
+    const double Cx = nuc_xyz[3 * wk.nucleus], Cy = nuc_xyz[3 * wk.nucleus + 1], Cz = nuc_xyz[3 * wk.nucleus + 2];
+    const double CAx = Cx - A.x, CAy = Cy - A.y, CAz = Cz - A.z, CBx = Cx - B.x, CBy = Cy - B.y, CBz = Cz - B.z;
+    const double CA2 = CAx * CAx + CAy * CAy + CAz * CAz, CB2 = CBx * CBx + CBy * CBy + CBz * CBz;
+    const double kx = -2.0 * (aA * CAx + aB * CBx), ky = -2.0 * (aA * CAy + aB * CBy), kz = -2.0 * (aA * CAz + aB * CBz);
+    const double k = sqrt(kx * kx + ky * ky + kz * kz);
+    const double pref = 4 * M_PI * gnorm(la, aA) * gnorm(lb, aB);
+    const double *Ug = U + (size_t) wk.nucleus * NPTS;
+    // ---- radial table: thread e < L1^2 owns entry (lam = e / L1, N = e % L1) ----
+    const int lam_e = tid / L1, n_e = tid % L1;
+    const bool own = tid < L1 * L1;
+    double s6 = 0.0, s7 = 0.0, s8 = 0.0;
+    for (int p0 = 0; p0 < NPTS; p0 += 64) {
+        if (tid < 64) {
+            const int i = p0 + tid;
+            const double r = T.r[i];
+            const int lev = i < 32 ? 0 : 31 - __clz(i) - 4;   // level of the point (level l >= 1: [16 << l, 32 << l))
+            const double w = i < 2048 ? T.w[i] * ldexp(1.0, lev - 6) : T.w[i];
+            const double power = k * r - (aA + aB) * r * r - aA * CA2 - aB * CB2;
+            Gc[tid] = (w * Ug[i]) * (pref * exp(power));
+        }
+        for (int e = tid; e < L1 * 64; e += blockDim.x) {
+            const int l = e / 64, i = p0 + e % 64;
+            Kc[e] = bessel_k(T.bes, T.rfun, l, T.r[i] * k);
+            Rc[e] = T.rpow[(size_t) l * NPTS + i];
+        }
+        __syncthreads();
+        if (own) {
+            double acc = 0.0;
+            for (int q = 0; q < 64; q++) acc += Gc[q] * Rc[n_e * 64 + q] * Kc[lam_e * 64 + q];
+            if (p0 < 2048) s6 += acc;
+            else if (p0 < 4096) s7 += acc;
+            else s8 += acc;
+        }
+        __syncthreads();
+    }
+    if (own) {
+        const double S6 = s6, S7 = 0.5 * S6 + s7, S8 = 0.5 * S7 + s8;
+        Q[tid] = fabs(S7 - S6) <= tol ? S7 : S8;
+    }
+    // ---- real spherical harmonics of k (spherical_harmonics.c:172-230) ----
+    if (tid < L1 * L1) {
+        const int lam = (int) sqrt((double) tid), mu = tid - lam * lam - lam;
+        double ux = 0.0, uy = 0.0, uz = 0.0;
+        if (k > 1e-14) {
+            ux = kx / k;
+            uy = ky / k;
+            uz = kz / k;
+        }
+        const double *cf = T.rsh + T.rsh_off[lam];
+        const int *cm = T.comb + T.comb_off[lam];
+        const int nc = (lam + 1) * (lam + 2) / 2;
+        double v = 0.0;
+        for (int c = 0; c < nc; c++)
+            v += cf[(mu + lam) * nc + c] * (rep_pow(ux, cm[3 * c]) * rep_pow(uy, cm[3 * c + 1]) * rep_pow(uz, cm[3 * c + 2]));
+        SY[tid] = v;
+    }
+    __syncthreads();
+    // ---- Omega_lam(I, J, K) for every (I, J, K) with I + J + K <= la + lb ----
+    for (int e = tid; e < L1 * nijk; e += blockDim.x) {
+        const int lam = e / nijk, q = e % nijk;
+        double sum = 0.0;
+        for (int mu = -lam; mu <= lam; mu++)
+            sum += T.w1[(size_t) (lam * lam + mu + lam) * NUC_NIJK + q] * SY[lam * lam + mu + lam];
+        Om[e] = sum;
+    }
+    __syncthreads();
+    // ---- Cartesian sum (type1_integrals.c:199-262) ----
+    for (int e = tid; e < ncA * ncB; e += blockDim.x) {
+        int ax, ay, az, bx, by, bz;
+        dev_cart(la, e / ncB, ax, ay, az);
+        dev_cart(lb, e % ncB, bx, by, bz);
+        double chi = 0.0;
+        for (int a = 0; a <= ax; a++)
+            for (int b = 0; b <= ay; b++)
+                for (int c = 0; c <= az; c++)
+                    for (int d = 0; d <= bx; d++)
+                        for (int f1 = 0; f1 <= by; f1++)
+                            for (int f = 0; f <= bz; f++) {
+                                const double factor = dev_binom(ax, a) * dev_binom(ay, b) * dev_binom(az, c) *
+                                                      dev_binom(bx, d) * dev_binom(by, f1) * dev_binom(bz, f) *
+                                                      pow(CAx, ax - a) * pow(CAy, ay - b) * pow(CAz, az - c) *
+                                                      pow(CBx, bx - d) * pow(CBy, by - f1) * pow(CBz, bz - f);
+                                if (fabs(factor) < 1e-13) continue;
+                                const int N = a + b + c + d + f1 + f;
+                                const int q = simplex_idx(a + d, b + f1, c + f);
+                                double som = 0.0;
+                                for (int lam = 0; lam <= lab; lam++) {
+                                    const double qv = Q[lam * L1 + N];
+                                    if (fabs(qv) < 1e-16) continue;
+                                    som += Om[lam * nijk + q] * qv;
+                                }
+                                chi += factor * som;
+                            }
+        out[e] = cA * cB * chi;
+    }
+}
+
+static const void *nuc_kernel(void) { return (const void *) k_nuc_prim; }
+
+// out block of each item = sum of its slices in work order (nucleus, then primitive pair); mirrored
+// (B, A) blocks are the transposes (the operator is real symmetric)
+__global__ __launch_bounds__(256) void k_nuc_sum(const NucItem *__restrict__ items, const NucWork *__restrict__ work,
+                                                 DevJob J, const double *__restrict__ slices, double *__restrict__ out)
+{
+    const NucItem it = items[blockIdx.x];
+    const int ncA = dev_ncart(J.sh[it.a].L), ncB = dev_ncart(J.sh[it.b].L);
+    for (int e = threadIdx.x; e < ncA * ncB; e += blockDim.x) {
+        double v = 0.0;
+        for (int w = it.w0; w < it.w1; w++) v += slices[work[w].slice + e];
+        const int i = e / ncB, j = e % ncB;
+        out[it.off + (long long) i * it.stride + j] = v;
+        if (it.mirror) out[it.moff + (long long) j * it.stride + i] = v;
+    }
+}
+
+// Host driver: the nuclear attraction blocks of `nitem` shell pairs summed over `nnuc` nuclei of
+// model 1..5 (numeric radial quadrature), written to out_host (out_len doubles, row stride per item).
+extern "C" int eng_nuclear(int nshell, libgrpp_shell_t *const *shells, int nnuc, const double *nuc_xyz,
+                           const int *charge, int model, const double *params, double tol, int nitem,
+                           const eng_item_t *items, long long out_len, double *out_host)
+{
+    if (!eng_device_ok()) return 1;
+    for (int i = 0; i < nshell; i++)
+        if (shells[i]->L < 0 || shells[i]->L > NUC_LMAX) {
+            eng_set_error("nuclear attraction: shell %d has l = %d; the numeric nuclear models support l <= %d", i,
+                          shells[i]->L, NUC_LMAX);
+            return 2;
+        }
+    DevTables T;
+    if (device_tables(0, NUC_LMAX, &T)) return 1;
+    // shells: raw coefficients and exponents (nuclear_attraction.c uses shell->coeffs as given)
+    std::vector<DShell> sh(nshell);
+    std::vector<double> alpha, craw;
+    for (int i = 0; i < nshell; i++) {
+        DShell d{};
+        d.L = shells[i]->L;
+        d.nprim = shells[i]->num_primitives;
+        d.poff = (int) alpha.size();
+        d.x = shells[i]->origin[0];
+        d.y = shells[i]->origin[1];
+        d.z = shells[i]->origin[2];
+        for (int k = 0; k < d.nprim; k++) {
+            alpha.push_back(shells[i]->alpha[k]);
+            craw.push_back(shells[i]->coeffs[k]);
+        }
+        sh[i] = d;
+    }
+    // the model's potential on the grid, per nucleus, with the C library (as the reference)
+    const HostTables &H = host_tables();
+    std::vector<double> U((size_t) nnuc * NPTS);
+    for (int n = 0; n < nnuc; n++)
+        if (nuc_model_potential(model, (double) charge[n], params + 3 * n, NPTS, H.r.data(), U.data() + (size_t) n * NPTS)) {
+            eng_set_error("nuclear attraction: unknown nuclear model %d", model);
+            return 2;
+        }
+    std::vector<NucItem> it(nitem);
+    std::vector<NucWork> work;
+    long long slen = 0;
+    int lab_max = 0;
+    for (int q = 0; q < nitem; q++) {
+        const int a = items[q].a, b = items[q].b;
+        it[q] = {a, b, (int) work.size(), 0, items[q].off, items[q].moff, items[q].stride, items[q].mirror};
+        lab_max = std::max(lab_max, sh[a].L + sh[b].L);
+        const long long blk = (long long) ncart(sh[a].L) * ncart(sh[b].L);
+        for (int n = 0; n < nnuc; n++)
+            for (int ia = 0; ia < sh[a].nprim; ia++)
+                for (int ib = 0; ib < sh[b].nprim; ib++) {
+                    work.push_back({q, n, ia, ib, slen});
+                    slen += blk;
+                }
+        it[q].w1 = (int) work.size();
+    }
+    if (work.empty() || nitem == 0) {
+        memset(out_host, 0, (size_t) out_len * sizeof(double));
+        return 0;
+    }
+    const int L1 = lab_max + 1, nijk = L1 * (L1 + 1) * (L1 + 2) / 6;
+    const size_t lds = (size_t) (2 * L1 * 64 + 64 + 2 * L1 * L1 + L1 * nijk) * sizeof(double);
+    // one set of temporary device buffers for the call
+    std::vector<void *> bufs;
+    auto put = [&](const void *src, size_t bytes, void **dst) -> hipError_t {
+        hipError_t e = hipMalloc(dst, std::max<size_t>(bytes, 8));
+        if (e != hipSuccess) return e;
+        bufs.push_back(*dst);
+        return src ? hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+    };
+    DShell *d_sh = nullptr;
+    double *d_alpha = nullptr, *d_craw = nullptr, *d_xyz = nullptr, *d_U = nullptr, *d_slices = nullptr, *d_out = nullptr;
+    NucItem *d_it = nullptr;
+    NucWork *d_work = nullptr;
+    hipError_t e = put(sh.data(), sh.size() * sizeof(DShell), (void **) &d_sh);
+    if (e == hipSuccess) e = put(alpha.data(), alpha.size() * sizeof(double), (void **) &d_alpha);
+    if (e == hipSuccess) e = put(craw.data(), craw.size() * sizeof(double), (void **) &d_craw);
+    if (e == hipSuccess) e = put(nuc_xyz, (size_t) 3 * nnuc * sizeof(double), (void **) &d_xyz);
+    if (e == hipSuccess) e = put(U.data(), U.size() * sizeof(double), (void **) &d_U);
+    if (e == hipSuccess) e = put(it.data(), it.size() * sizeof(NucItem), (void **) &d_it);
+    if (e == hipSuccess) e = put(work.data(), work.size() * sizeof(NucWork), (void **) &d_work);
+    if (e == hipSuccess) e = put(nullptr, (size_t) slen * sizeof(double), (void **) &d_slices);
+    if (e == hipSuccess) e = put(nullptr, (size_t) out_len * sizeof(double), (void **) &d_out);
+    if (e == hipSuccess) e = hipMemset(d_out, 0, (size_t) out_len * sizeof(double));
+    if (e == hipSuccess) {
+        DevJob J{};
+        J.sh = d_sh;
+        J.alpha = d_alpha;
+        J.craw = d_craw;
+        hipLaunchKernelGGL(k_nuc_prim, dim3((unsigned) work.size()), dim3(256), lds, 0, d_work, d_it, J, T, d_xyz, d_U,
+                           tol, d_slices);
+        hipLaunchKernelGGL(k_nuc_sum, dim3((unsigned) nitem), dim3(256), 0, 0, d_it, d_work, J, d_slices, d_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out_host, d_out, (size_t) out_len * sizeof(double), hipMemcpyDeviceToHost);
+    for (void *b : bufs) (void) hipFree(b);
+    if (e != hipSuccess) {
+        eng_set_error("nuclear attraction: %s", hipGetErrorString(e));
+        return 1;
+    }
     return 0;
 }
 

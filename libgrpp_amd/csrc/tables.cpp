@@ -260,6 +260,25 @@ static void build(HostTables &T)
             }
     }
 
+    // ---- type-1 angular couplings of the numeric nuclear-attraction path, summed over (r, s, t)
+    //      in the reference's loop order so that each coupling rounds as there ----
+    T.w1.assign((size_t) (NUC_LAB + 1) * (NUC_LAB + 1) * NUC_NIJK, 0.0);
+    for (int lam = 0; lam <= NUC_LAB; lam++)
+        for (int mu = -lam; mu <= lam; mu++)
+            for (int s = 0; s <= NUC_LAB; s++)
+                for (int I = s; I >= 0; I--)
+                    for (int J = s - I; J >= 0; J--) {
+                        const int K = s - I - J;
+                        double sum2 = 0.0;
+                        for (int r = 0; r <= lam; r++)
+                            for (int q = 0; q <= lam; q++)
+                                for (int t = 0; t <= lam; t++)
+                                    if (r + q + t == lam) sum2 += rsh_coef(lam, mu, r, q, t) * unit_sphere(I + r, J + q, K + t);
+                        // simplex index of (I, J, K): layer s, then I, then J (as the device's simplex_idx(t, u, v))
+                        const int idx = s * (s + 1) * (s + 2) / 6 + I * (s + 1) - I * (I - 1) / 2 + J;
+                        T.w1[(size_t) (lam * lam + mu + lam) * NUC_NIJK + idx] = sum2;
+                    }
+
     // ---- OC-shell Cartesian -> real-harmonic coefficients (outercore_integrals.c:171-180,545-553) ----
     T.slm_off.assign(LMAX_SHELL + 2, 0);
     for (int L = 0; L <= LMAX_SHELL; L++) T.slm_off[L + 1] = T.slm_off[L] + (2 * L + 1) * ncart(L);

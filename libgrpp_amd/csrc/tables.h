@@ -14,12 +14,19 @@ constexpr int BOYS_NN = 40;
 constexpr int RSH_LMAX = 30;      // real-harmonic coefficient tables 0..RSH_LMAX
 constexpr int LMAX_SHELL = 10;    // basis L (incl. the L+1 shells of gradients)
 constexpr int LMAX_POT = 10;
-constexpr int MAXPOW = 10;        // r^n, n <= MAXPOW
+constexpr int MAXPOW = 20;        // r^n, n <= MAXPOW (numeric type-1: N <= la + lb)
 constexpr int ANG_SUM_MAX = 14;   // L + nA bound: unit-sphere integrals need (2(L+nA)+1)!! <= 30!!
+
+constexpr int NUC_LMAX = 6;        // nuclear attraction by the numeric radial quadrature: l <= 6 per shell
+constexpr int NUC_LAB = 2 * NUC_LMAX;
+constexpr int NUC_NIJK = (NUC_LAB + 1) * (NUC_LAB + 2) * (NUC_LAB + 3) / 6;   // (I, J, K), I + J + K <= NUC_LAB
 
 struct HostTables {
     std::vector<double> r, w;                 // [NPTS]
     std::vector<double> rpow;                 // [(MAXPOW+1) * NPTS]
+    // type-1 angular couplings sum_{r+s+t=lam} y_{lam mu}^{rst} int_sphere x^{I+r} y^{J+s} z^{K+t}
+    // (angular_integrals.c:35-56 without the harmonic of k), [(lam^2 + mu + lam) * NUC_NIJK + simplex(I,J,K)]
+    std::vector<double> w1;
     std::vector<double> bes;                  // [BES_NPT][BES_NL][5]
     std::vector<double> besp;                 // [BES_NPT][BES_NL][10]: Taylor coefficient k over k! (Horner form)
                                               // of K_l (0..4), then of K_{l-1} (K_1 for l = 0) (5..9)

@@ -59,6 +59,9 @@ def lib() -> C.CDLL:
         L.libgrpp_outercore_potential_integrals_gradient.argtypes = [PSHELL, PSHELL, dptr, C.c_int, C.POINTER(PPOT),
                                                                      C.POINTER(PSHELL), dptr, pp, pp, pp, pp]
         L.libgrpp_overlap_integrals.argtypes = [PSHELL, PSHELL, dptr]
+        L.libgrpp_nuclear_attraction_integrals.argtypes = [PSHELL, PSHELL, dptr, C.c_int, C.c_int, dptr, dptr]
+        L.libgrpp_batch_nuclear_attraction_integrals.argtypes = [C.c_int, C.POINTER(PSHELL), C.c_int, dptr,
+                                                                 C.POINTER(C.c_int), C.c_int, dptr, dptr]
         L.libgrpp_batch_full_grpp_integrals.restype = C.c_int
         L.libgrpp_batch_full_grpp_integrals.argtypes = [C.c_int, C.POINTER(PSHELL), C.c_int, dptr, C.POINTER(PGRPP),
                                                         C.c_int, dptr, dptr, dptr, dptr, C.POINTER(Stats)]
@@ -157,6 +160,33 @@ def batch_matrices_noalloc(sysm: System, flags: int = 0, stream=None):
     finally:
         o.close()
     return out[0], out[1], out[2], out[3], st.as_dict()
+
+
+NUCLEAR_MODEL_POINT_CHARGE, NUCLEAR_MODEL_CHARGED_BALL, NUCLEAR_MODEL_GAUSSIAN = 0, 1, 2
+NUCLEAR_MODEL_FERMI, NUCLEAR_MODEL_FERMI_BUBBLE, NUCLEAR_MODEL_POINT_CHARGE_NUMERICAL = 3, 4, 5
+
+
+def batch_nuclear_attraction(shells: List[Shell], nuclei, charges, model: int, params=None) -> np.ndarray:
+    """libgrpp_batch_nuclear_attraction_integrals: the dim x dim nuclear attraction matrix of a shell
+    list summed over the nuclei (xyz in bohr); params[n] = (R_rms,) or (c, a[, k]) per nucleus."""
+    f = factory()
+    sh = f.shells(shells)
+    nn = len(nuclei)
+    xyz = np.ascontiguousarray(np.asarray(nuclei, dtype=np.float64).reshape(-1))
+    z = (C.c_int * max(1, nn))(*[int(q) for q in charges])
+    p3 = np.zeros(3 * max(1, nn))
+    if params is not None:
+        for n, pr in enumerate(params):
+            p3[3 * n:3 * n + len(pr)] = pr
+    dim = sum(s.cart_size for s in shells)
+    V = np.zeros((dim, dim))
+    try:
+        rc = lib().libgrpp_batch_nuclear_attraction_integrals(len(shells), sh, nn, as_dptr(xyz), z, model,
+                                                              as_dptr(p3), as_dptr(V))
+        _check(rc, "libgrpp_batch_nuclear_attraction_integrals")
+    finally:
+        f.free_shells(sh, len(shells))
+    return V
 
 
 def batch_gradients(sysm: System, flags: int = 0):
@@ -409,6 +439,15 @@ class Api:
             self.f.f["delete_shell"](shs[i])
         self._free(a, b, None)
         return [m.reshape(A.cart_size, B.cart_size) for m in ms]
+
+    def nuclear(self, A, B, C, charge: int, model: int, params=()):
+        """libgrpp_nuclear_attraction_integrals (nuclear_attraction.c:54)"""
+        a, b = self.f.shell(A), self.f.shell(B)
+        m = np.zeros(A.cart_size * B.cart_size)
+        self.L.libgrpp_nuclear_attraction_integrals(a, b, darr(C), int(charge), int(model), darr(list(params) or [0.0]),
+                                                    as_dptr(m))
+        self._free(a, b, None)
+        return m.reshape(A.cart_size, B.cart_size)
 
     def full(self, A, B, C, g: GRPP):
         a, b, h = self.f.shell(A), self.f.shell(B), self.f.grpp(g)

@@ -233,3 +233,27 @@ def _per_call(self, what, A, B, Cpos, arg, D=None):
 
 
 CpuLib.per_call = _per_call
+
+
+# nuclear models (libgrpp.h:247-254): 0 point charge (analytic), 1 charged ball (R_rms), 2 Gaussian
+# (R_rms), 3 Fermi (c, a), 4 Fermi + bubble (c, a, k), 5 point charge by the numeric quadrature
+def _nuclear(self, A, B, Cpos, charge, model, params=(0.0, 0.0, 0.0), tol=1e-16):
+    """nuclear attraction block of shells A, B for a nucleus of charge Z at Cpos"""
+    from libgrpp_amd.abi import as_dptr, darr
+    a, b = self.fac.shell(A), self.fac.shell(B)
+    m = np.zeros(A.cart_size * B.cart_size)
+    pp = darr(list(params) + [0.0] * (4 - len(params)))
+    if self.kind == "oracle":
+        fn = self.lib.orc_nuclear_attraction
+        fn.restype = None
+        fn(a, b, darr(Cpos), C.c_int(charge), C.c_int(model), pp, C.c_double(tol), as_dptr(m))
+    else:
+        fn = self.lib.libgrpp_nuclear_attraction_integrals
+        fn.restype = None
+        fn(a, b, darr(Cpos), C.c_int(charge), C.c_int(model), pp, as_dptr(m))
+    self.fac.f["delete_shell"](a)
+    self.fac.f["delete_shell"](b)
+    return m.reshape(A.cart_size, B.cart_size)
+
+
+CpuLib.nuclear = _nuclear

@@ -148,4 +148,21 @@ double orc_build_matrices(int nshell, orc_shell_t **shells, int ncenter, const d
 double orc_build_gradients(int nshell, orc_shell_t **shells, int natom, const double *xyz, orc_grpp_t **grpps,
                            double *out, int nthreads);
 
+
+/* nuclear attraction (orc_nuclear.c) */
+void orc_type1_prim_n1(const orc_shell_t *A, double a, const orc_shell_t *B, double b, const double *C, double zeta,
+                       double *out);
+void orc_nuclear_attraction(const orc_shell_t *A, const orc_shell_t *B, const double *C, int charge, int model,
+                            const double *params, double tol, double *M);
+double orc_fermi_sk(int k, double x);
+double orc_fermi_sk_table_value(int i, int col);
+double orc_potential_point(double r, double Z);
+double orc_potential_ball(double r, double Z, double R_rms);
+double orc_potential_gaussian(double r, double Z, double R_rms);
+double orc_potential_fermi(double r, double Z, double c, double a);
+double orc_potential_fermi_bubble(double r, double Z, double c, double a, double k);
+double orc_rms_johnson_1985(int A);
+double orc_rms_golovko_2008(int A);
+int orc_fermi_params(double R_rms, double *c, double *a);
+
 #endif

@@ -219,6 +219,14 @@ static void mmd_contracted(const orc_shell_t *A, const orc_shell_t *B, const dou
     free(buf);
 }
 
+/* one normalised primitive pair with an r^-1 exp(-zeta r_C^2) term (the point-charge nuclear
+ * attraction with zeta = 0, nuclear_attraction.c:73-84) */
+void orc_type1_prim_n1(const orc_shell_t *A, double a, const orc_shell_t *B, double b, const double *C, double zeta,
+                       double *out)
+{
+    mmd_prim(A, a, B, b, C, zeta, 1, out);
+}
+
 /* sum_k d_k <A| r^{n_k-2} exp(-z_k r_C^2) |B> (type1_integrals.c:43-102) */
 void orc_type1(const orc_shell_t *A, const orc_shell_t *B, const double *C, const orc_potential_t *pot, double *M)
 {

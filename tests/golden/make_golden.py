@@ -20,6 +20,10 @@ the reference sources; the fixtures are data only).
 * ``KCs/block_floor.npz``, ``KCs/gradient_block_floor.npz`` -- per-block floors of the KCs goldens;
 * ``high_l_percall_reference.npz`` -- per-call blocks of shells up to l = 8 (high_l_calls);
 * ``per_term_gradient_reference.npz`` -- the per-term gradient entry points on H2O tasks.
+* ``nuclear_reference.npz`` -- nuclear attraction blocks of all six nuclear models (nuclear_calls)
+  with floors, and a sample of the reference's Fermi S_k node table.
+* ``nuclear_molecule_reference.npz`` -- the nuclear attraction matrices of a three-nucleus
+  molecule (nuclear_molecule), every model, with per-block floors.
 * ``ThO/``, ``UO2/`` -- the reference's ThO and UO2 test cases (inputs, gzipped goldens) and the
   per-block floors of the reference's own arithmetic on them (``heavy_goldens``).
 
@@ -403,13 +407,121 @@ def high_l_calls():
     return out
 
 
+BOHR_FM = 52917.721   # fm per bohr
+
+
+def nuclear_calls():
+    """(name, A, B, C, charge, model, params) of the nuclear-attraction cases, seed 5: the six models
+    (nuclear_attraction.c:54-160) for U-238 with its Johnson radius and Fermi parameters, and with
+    radii blown up to a few tenths of a bohr so the finite size moves every block; shells up to
+    l = 6 with exponents to 1e6 (tight functions feel the nucleus), one centre on the nucleus or
+    none, contractions of one to three primitives"""
+    from libgrpp_amd import configs
+    rng = np.random.default_rng(5)
+    r_fm = 0.836 * 238 ** (1.0 / 3.0) + 0.570                       # Johnson 1985, fm
+    a_fm = 2.3 / (4 * np.log(3))
+    c_fm = np.sqrt(5.0 / 3.0 * r_fm ** 2 - 7.0 / 3.0 * np.pi ** 2 * a_fm ** 2)
+    real = {1: (r_fm / BOHR_FM,), 2: (r_fm / BOHR_FM,), 3: (c_fm / BOHR_FM, a_fm / BOHR_FM),
+            4: (c_fm / BOHR_FM, a_fm / BOHR_FM, 0.2), 0: (), 5: ()}
+    big = {1: (0.35,), 2: (0.35,), 3: (0.4, 0.06), 4: (0.4, 0.06, 0.7), 0: (), 5: ()}
+    pairs = [(0, 0, 3, 2), (1, 0, 2, 1), (2, 1, 2, 2), (3, 2, 1, 2), (4, 4, 2, 1), (6, 3, 1, 1), (6, 6, 1, 1)]
+    out = []
+    for la, lb, na, nb in pairs:
+        onnuc = (la + lb) % 2 == 0
+        A = configs._shell(rng, la, (0.0, 0.0, 0.0) if onnuc else (0.3, -0.2, 0.4), na, (0.2, 1.0e6))
+        B = configs._shell(rng, lb, (-0.25, 0.5, 0.1), nb, (0.1, 2.0e5))
+        for model in range(6):
+            for tag, prm in (("real", real[model]), ("big", big[model])):
+                if tag == "big" and model in (0, 5):
+                    continue
+                Z = 92 if tag == "real" else 20
+                out.append((f"m{model}_{tag}_{la}{lb}", A, B, (0.0, 0.0, 0.0), Z, model, prm))
+    return out
+
+
+def nuclear_reference():
+    """nuclear_reference.npz: libgrpp_nuclear_attraction_integrals of the genuine reference on
+    nuclear_calls(), each call's noise floor (contraction-off and reassociated builds), and a
+    sample of the Fermi S_k node table the reference ships as generated source
+    (specfunc_fermi_sk.c:129, read as text)"""
+    ref, nof, aso = _reference_builds()
+    out = {}
+    for name, A, B, Cpos, Z, model, prm in nuclear_calls():
+        R = ref.nuclear(A, B, Cpos, Z, model, prm)
+        fl = max(np.abs(R - nof.nuclear(A, B, Cpos, Z, model, prm)).max(),
+                 np.abs(R - aso.nuclear(A, B, Cpos, Z, model, prm)).max())
+        out[name] = R
+        out[name + "_floor"] = np.array(fl)
+    sk = parse_c_table(os.path.join(REF, "specfunc_fermi_sk.c"), "specfunc_fermi_sk_table[2001][10] = {", (2001, 10))
+    rng = np.random.default_rng(7)
+    si = np.concatenate([rng.integers(0, 2001, 300), [0, 1, 1000, 1999, 2000]])
+    sc = np.concatenate([rng.integers(0, 10, 300), [0, 9, 4, 2, 9]])
+    out["sk_i"], out["sk_col"], out["sk_value"] = si, sc, sk[si, sc]
+    np.savez_compressed(os.path.join(HERE, "nuclear_reference.npz"), **out)
+
+
+NUC_MOLECULE_PARAMS = {0: None, 1: [(0.3,), (0.1,), (0.2,)], 2: [(0.3,), (0.1,), (0.2,)],
+                       3: [(0.4, 0.06), (0.2, 0.05), (0.3, 0.04)],
+                       4: [(0.4, 0.06, 0.7), (0.2, 0.05, 0.3), (0.3, 0.04, 0.1)], 5: None}
+
+
+def nuclear_molecule():
+    """(shells, nuclei, charges) of a three-nucleus molecule, seed 11: twelve shells up to l = 4
+    with two primitives of exponents 0.3 ... 200 on the nuclei.  Tighter functions off a nucleus
+    leave the reference's numeric models ill-conditioned: with exponents to 5e4 its builds differ
+    by 3e8 on elements of 4e8"""
+    from libgrpp_amd import configs
+    rng = np.random.default_rng(11)
+    nuclei = [(0.0, 0.0, 0.0), (0.0, 0.0, 2.1), (1.6, -0.7, 0.4)]
+    shells = []
+    for c, ls in zip(nuclei, ([0, 0, 1, 2, 3], [0, 1, 1, 2], [0, 1, 4])):
+        for l in ls:
+            shells.append(configs._shell(rng, l, c, 2, (0.3, 200.0)))
+    return shells, nuclei, [30, 8, 14]
+
+
+def _nuc_mol_job(job):
+    model, i, j = job
+    shells, nuclei, Z = nuclear_molecule()
+    prm = NUC_MOLECULE_PARAMS[model]
+    builds = _reference_builds()
+    blk, fl = 0.0, 0.0
+    for n, Cp in enumerate(nuclei):
+        R, N, A = [b.nuclear(shells[i], shells[j], Cp, Z[n], model, prm[n] if prm else ()) for b in builds]
+        blk = blk + R
+        fl += max(np.abs(R - N).max(), np.abs(R - A).max())
+    return model, i, j, blk, fl
+
+
+def nuclear_molecule_reference():
+    """nuclear_molecule_reference.npz: per model, the dim x dim matrix of the reference's per-call
+    blocks summed over the nuclei (every ordered shell pair) and floor[ns, ns], the sum over the
+    nuclei of each block's noise floor"""
+    from multiprocessing import Pool
+    shells, nuclei, Z = nuclear_molecule()
+    ns = len(shells)
+    off = np.cumsum([0] + [s.cart_size for s in shells])
+    jobs = [(m, i, j) for m in range(6) for i in range(ns) for j in range(ns)]
+    with Pool(os.cpu_count()) as pool:
+        res = pool.map(_nuc_mol_job, jobs, chunksize=1)
+    out = {}
+    for m in range(6):
+        out[f"V{m}"] = np.zeros((off[-1], off[-1]))
+        out[f"floor{m}"] = np.zeros((ns, ns))
+    for m, i, j, blk, fl in res:
+        out[f"V{m}"][off[i]:off[i + 1], off[j]:off[j + 1]] = blk
+        out[f"floor{m}"][i, j] = fl
+    np.savez_compressed(os.path.join(HERE, "nuclear_molecule_reference.npz"), **out)
+
+
 OPS = ["arep", "so_x", "so_y", "so_z"]
 
 
 FIXTURES = {"tables": tables_sample, "small_pair": small_pair_reference, "h2o_pergrad": h2o_pergrad_reference,
             "heavy_cluster": heavy_cluster_reference, "ragged": ragged_reference, "heavy_goldens": heavy_goldens,
             "compact": compact_reference, "kcs_floors": lambda: (heavy_goldens(("KCs",)), kcs_gradient_floors()),
-            "high_l": high_l_percall, "per_term_gradients": per_term_gradients}
+            "high_l": high_l_percall, "per_term_gradients": per_term_gradients, "nuclear": nuclear_reference,
+            "nuclear_molecule": nuclear_molecule_reference}
 
 
 if __name__ == "__main__":
