@@ -39,6 +39,11 @@ def build(verbose: bool = False) -> str:
     steps = [
         ([HIPCC] + HIP_FLAGS + inc + ["-c", os.path.join(CSRC, "engine.hip"), "-o", os.path.join(BUILD, "engine.o")],
          os.path.join(BUILD, "engine.o")),
+        # the screening, grid-tabulation and contraction kernels: without machine loop-invariant code
+        # motion, which hoisted their FP64 constants out of the loops and spilled them
+        ([HIPCC] + HIP_FLAGS + ["-mllvm", "-disable-machine-licm"] + inc +
+         ["-c", os.path.join(CSRC, "valu_kernels.hip"), "-o", os.path.join(BUILD, "valu_kernels.o")],
+         os.path.join(BUILD, "valu_kernels.o")),
         ([HIPCC] + HIP_FLAGS + inc + ["-c", os.path.join(CSRC, "mlp.hip"), "-o", os.path.join(BUILD, "mlp.o")],
          os.path.join(BUILD, "mlp.o")),
         (["g++", "-O3", "-fPIC", "-std=c++17"] + inc + ["-c", os.path.join(CSRC, "tables.cpp"), "-o",
