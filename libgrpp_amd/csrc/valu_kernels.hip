@@ -425,7 +425,7 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
                                                   int nm_max, int qs_len, int ts_len)
 {
     constexpr int CT_EPT = 256 / CT_THREADS;
-    // dynamic LDS sized by the plan: Q block (qs_len doubles) + (m, i, k2) intermediate (ts_len)
+    // dynamic LDS sized by the plan: Q block (qs_len doubles) + (i, m, k2) intermediate (ts_len)
     extern __shared__ double ct_lds[];
     double *Qs = ct_lds, *Ts = ct_lds + qs_len;
     const PartItem pt = partlist[blockIdx.x];
@@ -436,7 +436,7 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
     const int la = sa.L, lb = sb.L, ncA = dev_ncart(la), ncB = dev_ncart(lb);
     const int tid = threadIdx.x;
     const double FOURPI2 = 16.0 * M_PI * M_PI;
-    // rows of A per pass: the outputs fit CT_EPT per thread and T[m][i][k2] fits TS_MAX
+    // rows of A per pass: the outputs fit CT_EPT per thread and T[i][m][k2] fits TS_MAX
     const int rows_per_chunk = max(1, min((CT_EPT * CT_THREADS) / ncB, ts_len / (nm_max * ncB)));
     for (int r0 = 0; r0 < ncA; r0 += rows_per_chunk) {
         const int r1 = min(ncA, r0 + rows_per_chunk), nr = r1 - r0, nel = nr * ncB;
@@ -498,10 +498,14 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
                 const double *PB = J.P + J.slot_poff[(long long) p.slot * J.nshell + it.b];
                 const double *PLB = is_so ? J.P + J.slot_ploff[(long long) p.slot * J.nshell + it.b] : nullptr;
                 const double fac = FOURPI2 * p.scale;
-                // T[m][i][k2] = sum_k1 P^A[i][m][k1] Q[k1][k2] for all m at once
-                for (int e = tid; e < nm * nr * KB; e += blockDim.x) {
-                    const int k2 = e % KB, mi = e / KB, i = mi % nr, m = mi / nr;
-                    const double *pa = PA + ((long long) (r0 + i) * nm + m) * KA;
+                // T[i][m][k2] = sum_k1 P^A[i][m][k1] Q[k1][k2] for all m at once; row i of T is then
+                // contiguous over (m, k2), as row j of the ket projectors is, so each output is one
+                // dot product of length nm * KB (the same terms in the same order as a loop over m
+                // and k2)
+                const int nmk = nm * KB;
+                for (int e = tid; e < nr * nmk; e += blockDim.x) {
+                    const int im = e / KB, k2 = e - im * KB;   // im = i * nm + m
+                    const double *pa = PA + ((long long) r0 * nm + im) * KA;
                     double sacc = 0.0;
 #pragma unroll 4
                     for (int k1 = 0; k1 < KA; k1++) sacc += pa[k1] * Qs[k1 * KB + k2];
@@ -512,26 +516,21 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
                 for (int q = 0; q < CT_EPT; q++) {
                     const int e = tid + q * CT_THREADS;
                     if (e >= nel) break;
-                    const int i = e / ncB, j = e % ncB;
+                    const int i = e / ncB, j = e - i * ncB;
+                    const double *tr = Ts + i * nmk;
                     if (!is_so) {
+                        const double *pb = PB + (long long) j * nmk;
                         double sacc = 0.0;
-                        for (int m = 0; m < nm; m++) {
-                            const double *tr = Ts + (m * nr + i) * KB;
-                            const double *pb = PB + ((long long) j * nm + m) * KB;
-#pragma unroll 4
-                            for (int k2 = 0; k2 < KB; k2++) sacc += tr[k2] * pb[k2];
-                        }
+#pragma unroll 8
+                        for (int x = 0; x < nmk; x++) sacc += tr[x] * pb[x];
                         acc[0][q] += fac * sacc;
                     } else {
 #pragma unroll
                         for (int eta = 0; eta < 3; eta++) {
+                            const double *pl = PLB + ((long long) eta * ncB + j) * nmk;
                             double sacc = 0.0;
-                            for (int m = 0; m < nm; m++) {
-                                const double *tr = Ts + (m * nr + i) * KB;
-                                const double *pl = PLB + (((long long) eta * ncB + j) * nm + m) * KB;
-#pragma unroll 4
-                                for (int k2 = 0; k2 < KB; k2++) sacc += tr[k2] * pl[k2];
-                            }
+#pragma unroll 8
+                            for (int x = 0; x < nmk; x++) sacc += tr[x] * pl[x];
                             acc[1 + eta][q] += fac * sacc;
                         }
                     }
