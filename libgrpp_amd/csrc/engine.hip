@@ -445,11 +445,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 // ================================================================== angular projectors
 constexpr int PROJ_LMAX = LMAX_POT + LMAX_SHELL;  // lambda <= L + l
 
+// Om (the Omega block of one n) is dynamic LDS of the plan's largest size, om_len doubles
 __global__ __launch_bounds__(256) void k_proj(const ProjItem *__restrict__ items, DevJob J, DevTables T,
                                               double *__restrict__ P)
 {
     __shared__ double Sk[(PROJ_LMAX + 1) * (2 * PROJ_LMAX + 1)];
-    __shared__ double Om[6144];
+    __shared__ double CApow[3][LMAX_SHELL + 1];   // pow(CA_d, k), k <= l_A (type2_integrals.c:132-142)
+    extern __shared__ double Om[];
     const ProjItem it = items[blockIdx.x];
     const DShell s = J.sh[it.A];
     const DCenter c = J.ctr[it.c];
@@ -468,6 +470,10 @@ __global__ __launch_bounds__(256) void k_proj(const ProjItem *__restrict__ items
         uz = kz * inv;
     }
     const int lmx = L + la;
+    if (tid < 3 * (la + 1)) {
+        const int d = tid / (la + 1), k = tid - d * (la + 1);
+        CApow[d][k] = pow(d == 0 ? CAx : d == 1 ? CAy : CAz, k);
+    }
     // real harmonics S_{lambda mu}(k) with the |S| < 1e-14 skip of angular_integrals.c:77
     for (int e = tid; e < (lmx + 1) * (2 * PROJ_LMAX + 1); e += blockDim.x) {
         const int lam = e / (2 * PROJ_LMAX + 1), mu = e % (2 * PROJ_LMAX + 1);
@@ -516,11 +522,11 @@ __global__ __launch_bounds__(256) void k_proj(const ProjItem *__restrict__ items
             dev_cart(la, i, cx, cy, cz);
             double v = 0.0, v2 = 0.0, v3 = 0.0;
             for (int a = 0; a <= cx && a <= n; a++) {
-                const double fa = dev_binom(cx, a) * pow(CAx, cx - a);
+                const double fa = dev_binom(cx, a) * CApow[0][cx - a];
                 for (int b = 0; b <= cy && a + b <= n; b++) {
                     const int cc = n - a - b;
                     if (cc > cz) continue;
-                    const double f = fa * dev_binom(cy, b) * pow(CAy, cy - b) * dev_binom(cz, cc) * pow(CAz, cz - cc);
+                    const double f = fa * dev_binom(cy, b) * CApow[1][cy - b] * dev_binom(cz, cc) * CApow[2][cz - cc];
                     const int q = (n - a) * (n - a + 1) / 2 + (n - a - b);
                     const double o = Om[(j * nm + m) * nab + q];
                     v += f * o;
@@ -1001,7 +1007,7 @@ static const void *nuc_kernel(void);
 static int raise_lds_limits(void)
 {
     const int cu_lds = 160 * 1024;
-    const void *fns[] = {(const void *) k_ocv, (const void *) k_type1, contract_kernel_ptr(64),
+    const void *fns[] = {(const void *) k_ocv, (const void *) k_type1, (const void *) k_proj, contract_kernel_ptr(64),
                          contract_kernel_ptr(128), contract_kernel_ptr(256),
                          nuc_kernel()};
     for (const void *f : fns) {
@@ -1169,6 +1175,7 @@ struct eng_plan {
     std::vector<TileItem> tiles_host;
     ProjItem *d_proj = nullptr;
     int n_proj = 0;
+    int proj_om = 1;                 // k_proj dynamic LDS doubles: the largest Omega block of one n
     OcvItem *d_ocvi = nullptr;
     int n_ocv = 0;
     int *d_delta_centers = nullptr;
@@ -1474,6 +1481,7 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     std::vector<long long> slot_pb2((size_t) nslot * nshell, -1), slot_pb3((size_t) nslot * nshell, -1);
     std::vector<ProjItem> proj;
     long long plen = 0;
+    int proj_om = 1;
     for (int s = 0; s < nslot; s++) {
         const int c = slots[s].c, L = slots[s].L;
         slot_rows[s] = (int) rows.size();
@@ -1514,6 +1522,8 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
             slot_pb2[(size_t) s * nshell + A] = pi.pb2;
             slot_pb3[(size_t) s * nshell + A] = pi.pb3;
             proj.push_back(pi);
+            for (int n = 0; n <= sh[A].L; n++)
+                proj_om = std::max(proj_om, lam_cnt(L, n) * (2 * L + 1) * (n + 1) * (n + 2) / 2);
         }
         slot_M[s] = M;
     }
@@ -1761,6 +1771,7 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     // 5120 screening wave slots (256 CUs x 4 SIMDs x 5 waves) about six waves each, else 4
     P->tune.screen_split = 4LL * P->n_tiles < 8 * 4096 ? 16 : 4;
     P->n_proj = (int) proj.size();
+    P->proj_om = proj_om;
     P->n_ocv = (int) ocv.size();
     P->n_delta_centers = (int) delta_centers.size();
     P->n_items = nitem;
@@ -1914,7 +1925,9 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
         launch_tab_F(P->n_tab, sd, P->d_tab, P->J, P->T, P->d_F);
     }
     HIPCHK(rec(P->ev[5], sd));
-    if (P->n_proj > 0) hipLaunchKernelGGL(k_proj, dim3(P->n_proj), dim3(256), 0, sd, P->d_proj, P->J, P->T, P->d_P);
+    if (P->n_proj > 0)
+        hipLaunchKernelGGL(k_proj, dim3(P->n_proj), dim3(256), (size_t) P->proj_om * sizeof(double), sd, P->d_proj,
+                           P->J, P->T, P->d_P);
     if (P->n_delta_centers > 0)
         hipLaunchKernelGGL(k_delta, dim3(P->n_delta_centers), dim3(64), 0, sd, P->d_delta_centers,
                            P->n_delta_centers, P->J, P->d_delta);
