@@ -184,6 +184,15 @@ int libgrpp_batch_gradients_plan_create(int nshells, libgrpp_shell_t *const *she
 long long libgrpp_batch_plan_gradient_len(const libgrpp_batch_plan_t *plan);   /* 12 * ncenters * dim^2 */
 int libgrpp_batch_plan_download_gradients(libgrpp_batch_plan_t *plan, double *out);
 /*
+ * The gradients of the listed atoms only (no counterpart in the reference): out holds
+ * 12 * natoms * dim^2 doubles, out[((op * 3 * natoms) + 3 * a + k) * dim^2 + i * dim + j] for the
+ * a-th atom of the list (each atom index in [0, ncenters), at most once), the same values as the
+ * full download.  A caller streams the gradients of a large system atom by atom instead of holding
+ * all 12 * ncenters * dim^2 doubles.
+ */
+int libgrpp_batch_plan_download_gradients_atoms(libgrpp_batch_plan_t *plan, int natoms, const int *atoms,
+                                                double *out);
+/*
  * Radial-grid level predictor hooks.  The plan's radial tables have one row per (centre, channel
  * L, shell, lambda, n_A).  features: *n rows x 12 floats (L, l, lambda, n, |CA|, ln alpha_min,
  * ln alpha_max, ln zeta_min, ln zeta_max, ln(1 + 2 alpha_max |CA|), primitives, on-centre flag);

@@ -901,22 +901,24 @@ long long libgrpp_batch_plan_gradient_len(const libgrpp_batch_plan_t *p)
     return 12LL * p->grad->ncenters * p->dim * p->dim;
 }
 
-int libgrpp_batch_plan_download_gradients(libgrpp_batch_plan_t *p, double *out)
+/* the scatter of the engine's blocks into out: slot[atom] is the position of atom's gradients in
+ * out (nslot atoms), -1 for atoms that are not downloaded */
+static int download_gradients_slots(libgrpp_batch_plan_t *p, int nslot, const int *slot, double *out)
 {
     const grad_host_t *g = p->grad;
-    if (!g) {
-        eng_set_error("download_gradients: the plan was not created by libgrpp_batch_gradients_plan_create");
-        return 2;
-    }
-    const int nshells = g->nshells, ncenters = g->ncenters, nidx = g->nidx, dim = p->dim;
+    const int nshells = g->nshells, nidx = g->nidx, dim = p->dim;
     const long long n2 = (long long) dim * dim, tot = g->tot;
     double *buf = (double *) malloc(sizeof(double) * 4 * (tot > 0 ? tot : 1));
+    if (!buf) {
+        eng_set_error("download_gradients: out of host memory for %lld doubles", 4 * tot);
+        return 1;
+    }
     int rc = eng_plan_download(p->eng, buf);
     if (rc) {
         free(buf);
         return rc;
     }
-    memset(out, 0, sizeof(double) * 12 * ncenters * n2);
+    memset(out, 0, sizeof(double) * 12 * (size_t) nslot * n2);
     int maxnc = 1, maxnu = 1;
     for (int i = 0; i < nshells; i++) {
         if (g->nc[i] > maxnc) maxnc = g->nc[i];
@@ -934,6 +936,8 @@ int libgrpp_batch_plan_download_gradients(libgrpp_batch_plan_t *p, double *out)
                 const double *C = g->xyz + 3 * c;
                 const int AC = points_equal(g->origin + 3 * i, C), BC = points_equal(g->origin + 3 * j, C);
                 if (AC && BC) continue;
+                if (slot[c] < 0 && (g->atom[i] < 0 || slot[g->atom[i]] < 0) && (g->atom[j] < 0 || slot[g->atom[j]] < 0))
+                    continue;   /* touches no downloaded atom */
                 const int nA = g->nc[i], nB = g->nc[j];
                 const long long kij = ((long long) i * nshells + j) * nidx + ci;
                 const long long kji = ((long long) j * nshells + i) * nidx + ci;
@@ -961,19 +965,21 @@ int libgrpp_batch_plan_download_gradients(libgrpp_batch_plan_t *p, double *out)
                 for (int op = 0; op < 4; op++)
                     for (int k = 0; k < 3; k++) {
                         const double *br = bra + (op * 3 + k) * nA * nB, *kk = ket + (op * 3 + k) * nA * nB;
-                        double *gA = g->atom[i] >= 0 ? out + ((long long) (op * 3 * ncenters + 3 * g->atom[i] + k)) * n2 : NULL;
-                        double *gB = g->atom[j] >= 0 ? out + ((long long) (op * 3 * ncenters + 3 * g->atom[j] + k)) * n2 : NULL;
-                        double *gC = out + ((long long) (op * 3 * ncenters + 3 * c + k)) * n2;
+                        const int sa = g->atom[i] >= 0 ? slot[g->atom[i]] : -1;
+                        const int sb = g->atom[j] >= 0 ? slot[g->atom[j]] : -1;
+                        double *gA = sa >= 0 ? out + ((long long) (op * 3 * nslot + 3 * sa + k)) * n2 : NULL;
+                        double *gB = sb >= 0 ? out + ((long long) (op * 3 * nslot + 3 * sb + k)) * n2 : NULL;
+                        double *gC = slot[c] >= 0 ? out + ((long long) (op * 3 * nslot + 3 * slot[c] + k)) * n2 : NULL;
                         for (int a = 0; a < nA; a++)
                             for (int b = 0; b < nB; b++) {
                                 const long long e = (long long) (g->off[i] + a) * dim + g->off[j] + b;
                                 if (!AC) {
                                     if (gA) gA[e] += br[a * nB + b];
-                                    gC[e] -= br[a * nB + b];
+                                    if (gC) gC[e] -= br[a * nB + b];
                                 }
                                 if (!BC) {
                                     if (gB) gB[e] += kk[a * nB + b];
-                                    gC[e] -= kk[a * nB + b];
+                                    if (gC) gC[e] -= kk[a * nB + b];
                                 }
                             }
                     }
@@ -985,6 +991,46 @@ int libgrpp_batch_plan_download_gradients(libgrpp_batch_plan_t *p, double *out)
     free(td);
     free(buf);
     return 0;
+}
+
+int libgrpp_batch_plan_download_gradients(libgrpp_batch_plan_t *p, double *out)
+{
+    if (!p->grad) {
+        eng_set_error("download_gradients: the plan was not created by libgrpp_batch_gradients_plan_create");
+        return 2;
+    }
+    const int n = p->grad->ncenters;
+    int *slot = (int *) malloc(sizeof(int) * (n > 0 ? n : 1));
+    for (int c = 0; c < n; c++) slot[c] = c;
+    const int rc = download_gradients_slots(p, n, slot, out);
+    free(slot);
+    return rc;
+}
+
+int libgrpp_batch_plan_download_gradients_atoms(libgrpp_batch_plan_t *p, int natoms, const int *atoms, double *out)
+{
+    if (!p->grad) {
+        eng_set_error("download_gradients_atoms: the plan was not created by libgrpp_batch_gradients_plan_create");
+        return 2;
+    }
+    const int n = p->grad->ncenters;
+    if (natoms < 0 || (natoms > 0 && !atoms)) {
+        eng_set_error("download_gradients_atoms: invalid atom list");
+        return 2;
+    }
+    int *slot = (int *) malloc(sizeof(int) * (n > 0 ? n : 1));
+    for (int c = 0; c < n; c++) slot[c] = -1;
+    for (int a = 0; a < natoms; a++) {
+        if (atoms[a] < 0 || atoms[a] >= n || slot[atoms[a]] >= 0) {
+            eng_set_error("download_gradients_atoms: atom %d out of range [0, %d) or listed twice", atoms[a], n);
+            free(slot);
+            return 2;
+        }
+        slot[atoms[a]] = a;
+    }
+    const int rc = download_gradients_slots(p, natoms, slot, out);
+    free(slot);
+    return rc;
 }
 
 int libgrpp_batch_full_grpp_gradients(int nshells, libgrpp_shell_t *const *shells, int ncenters, const double *xyz,

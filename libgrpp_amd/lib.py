@@ -91,6 +91,7 @@ def lib() -> C.CDLL:
         L.libgrpp_batch_plan_gradient_len.restype = C.c_longlong
         L.libgrpp_batch_plan_gradient_len.argtypes = [C.c_void_p]
         L.libgrpp_batch_plan_download_gradients.argtypes = [C.c_void_p, dptr]
+        L.libgrpp_batch_plan_download_gradients_atoms.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), dptr]
         L.libgrpp_init()
         _lib = L
     return _lib
@@ -377,6 +378,15 @@ class GradientPlan(_LevelHooks):
         out = np.zeros((4, 3 * na, dim, dim))
         assert lib().libgrpp_batch_plan_gradient_len(self.h) == out.size
         _check(lib().libgrpp_batch_plan_download_gradients(self.h, as_dptr(out)), "libgrpp_batch_plan_download_gradients")
+        return out
+
+    def download_atoms(self, atoms) -> np.ndarray:
+        """the gradients of the listed atoms only: grad[op, 3*a+k] for the a-th atom of the list"""
+        dim = self.sysm.dim
+        at = (C.c_int * len(atoms))(*[int(a) for a in atoms])
+        out = np.zeros((4, 3 * len(atoms), dim, dim))
+        _check(lib().libgrpp_batch_plan_download_gradients_atoms(self.h, len(atoms), at, as_dptr(out)),
+               "libgrpp_batch_plan_download_gradients_atoms")
         return out
 
     def stats(self):

@@ -24,6 +24,10 @@ the reference sources; the fixtures are data only).
   with floors, and a sample of the reference's Fermi S_k node table.
 * ``nuclear_molecule_reference.npz`` -- the nuclear attraction matrices of a three-nucleus
   molecule (nuclear_molecule), every model, with per-block floors.
+* ``high_l_cluster_reference.npz`` -- full-GRPP blocks of sampled ordered pairs (l_A + l_B <= 4, all
+  32 centres) of high_l_cluster at its stated size, with floors; ``high_l_cluster_gradient_reference.npz``
+  -- the reference's per-task gradients (centre 0's GRPP, with respect to the atoms of A, B and the
+  centre) of sampled pairs of the same system, with floors.
 * ``ThO/``, ``UO2/`` -- the reference's ThO and UO2 test cases (inputs, gzipped goldens) and the
   per-block floors of the reference's own arithmetic on them (``heavy_goldens``).
 
@@ -517,11 +521,102 @@ def nuclear_molecule_reference():
 OPS = ["arep", "so_x", "so_y", "so_z"]
 
 
+def _high_l_job(job):
+    """one ordered pair of high_l_cluster (all 32 centres) through the three reference builds, one thread"""
+    from libgrpp_amd import configs
+    return job, _sampled_blocks([configs.high_l_cluster()], [(0, [job])], nthreads=1)
+
+
+def high_l_cluster_reference(npair: int = 32, lsum_max: int = 4):
+    """high_l_cluster at its stated size (32 atoms in a 20-bohr cube, GRPP L <= 5 + local, SO L <= 5,
+    outercore s/p/d, one shell of every l <= 8 per atom with 8 primitives; seed 0): npair random
+    ordered shell pairs with l_A + l_B <= lsum_max, each with all 32 centres, through the genuine
+    reference and its two other roundings (floors as in _sampled_blocks).  The bound keeps the
+    reference's cost in minutes (4-12 s per pair and build)."""
+    from multiprocessing import Pool
+    from libgrpp_amd import configs
+    sysm = configs.high_l_cluster()
+    ns = len(sysm.shells)
+    rng = np.random.default_rng(2028)
+    ok = [int(p) for p in rng.permutation(ns * ns) if sysm.shells[p // ns].L + sysm.shells[p % ns].L <= lsum_max]
+    jobs = sorted(ok[:npair])
+    with Pool(os.cpu_count()) as pool:
+        res = pool.map(_high_l_job, jobs, chunksize=1)
+    offs, vals, floor = [0], [], []
+    for p, d in res:
+        vals.append(d["ref"])
+        floor.append(d["floor"][0])
+        offs.append(offs[-1] + d["ref"].size)
+    np.savez_compressed(os.path.join(HERE, "high_l_cluster_reference.npz"), molecule=np.zeros(len(jobs), np.int32),
+                        pair=np.array(jobs, np.int32), offset=np.array(offs, np.int64), ref=np.concatenate(vals),
+                        floor=np.array(floor))
+
+
+def _high_l_grad_job(job):
+    """libgrpp_full_grpp_integrals_gradient of the three reference builds for one (a, b, point d) task
+    of high_l_cluster with centre 0's GRPP: the 12 derivative blocks of each build"""
+    import oracle
+    from libgrpp_amd import configs
+    from libgrpp_amd.abi import PGRPP, PSHELL, as_dptr, darr, dptr
+    a, b, d = job
+    sysm = configs.high_l_cluster()
+    atoms = [c for c, _ in sysm.centers]
+    A, B = sysm.shells[a], sysm.shells[b]
+    na, nb = A.cart_size, B.cart_size
+    res = []
+    for ref in _reference_builds():
+        L = ref.lib
+        pp = C.POINTER(dptr)
+        L.libgrpp_full_grpp_integrals_gradient.argtypes = [PSHELL, PSHELL, PGRPP, dptr, dptr, pp, pp, pp, pp]
+        sa, sb = ref.fac.shell(A), ref.fac.shell(B)
+        h = ref.fac.grpp(sysm.centers[0][1])
+        arrs = [[np.zeros(na * nb) for _ in range(3)] for _ in range(4)]
+        ptrs = [(dptr * 3)(*[as_dptr(x) for x in ar]) for ar in arrs]
+        L.libgrpp_full_grpp_integrals_gradient(sa, sb, h, darr(atoms[0]), darr(atoms[d]), *ptrs)
+        res.append(np.array([[x.reshape(na, nb) for x in ar] for ar in arrs]))
+        ref.fac.free_grpp(h)
+    return job, res
+
+
+def high_l_gradient_reference(npair: int = 12, lsum_max: int = 3, natom: int = 4):
+    """high_l_cluster_gradient_reference.npz: for npair random ordered shell pairs (l_A + l_B <=
+    lsum_max, both shells on the first natom atoms, so that a test streams the gradients of natom
+    atoms only) of high_l_cluster at its stated size, the reference's libgrpp_full_grpp_integrals_gradient
+    with centre 0's GRPP, differentiated with respect to the atoms of A, of B and atom 0 (the GRPP
+    centre): values [op, k] of each (a, b, d) task and their floor per operator (largest move of the
+    contraction-off and reassociation builds)"""
+    from multiprocessing import Pool
+    from libgrpp_amd import configs
+    sysm = configs.high_l_cluster()
+    ns = len(sysm.shells)
+    atoms = [c for c, _ in sysm.centers]
+    atom_of = [next(i for i, c in enumerate(atoms) if c == s.origin) for s in sysm.shells]
+    rng = np.random.default_rng(2029)
+    ok = [int(p) for p in rng.permutation(ns * ns) if sysm.shells[p // ns].L + sysm.shells[p % ns].L <= lsum_max
+          and atom_of[p // ns] < natom and atom_of[p % ns] < natom]
+    jobs = []
+    for p in sorted(ok[:npair]):
+        a, b = divmod(p, ns)
+        for d in sorted({atom_of[a], atom_of[b], 0}):
+            jobs.append((a, b, d))
+    with Pool(os.cpu_count()) as pool:
+        res = pool.map(_high_l_grad_job, jobs, chunksize=1)
+    tasks, offs, vals, floor = [], [0], [], []
+    for (a, b, d), (R, N, A) in res:
+        tasks.append((a, b, d))
+        vals.append(R.ravel())
+        floor.append([max(np.abs(R[op] - N[op]).max(), np.abs(R[op] - A[op]).max()) for op in range(4)])
+        offs.append(offs[-1] + R.size)
+    np.savez_compressed(os.path.join(HERE, "high_l_cluster_gradient_reference.npz"), tasks=np.array(tasks, np.int32),
+                        offset=np.array(offs, np.int64), ref=np.concatenate(vals), floor=np.array(floor))
+
+
 FIXTURES = {"tables": tables_sample, "small_pair": small_pair_reference, "h2o_pergrad": h2o_pergrad_reference,
             "heavy_cluster": heavy_cluster_reference, "ragged": ragged_reference, "heavy_goldens": heavy_goldens,
             "compact": compact_reference, "kcs_floors": lambda: (heavy_goldens(("KCs",)), kcs_gradient_floors()),
             "high_l": high_l_percall, "per_term_gradients": per_term_gradients, "nuclear": nuclear_reference,
-            "nuclear_molecule": nuclear_molecule_reference}
+            "nuclear_molecule": nuclear_molecule_reference, "high_l_cluster": high_l_cluster_reference,
+            "high_l_gradients": high_l_gradient_reference}
 
 
 if __name__ == "__main__":

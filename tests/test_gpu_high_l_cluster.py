@@ -1,0 +1,168 @@
+"""The high angular momentum class at its stated size: configs.high_l_cluster() defaults, 32 heavy
+centres in a 20-bohr cube (minimum separation 3 bohr), each with a GRPP of semilocal L <= 5 plus
+local, spin-orbit L <= 5 and outercore s/p/d projectors, and one Cartesian shell of every l <= 8 per
+atom with 8 primitives (exponents log-uniform in [0.02, 80]): 288 shells, dim 5280, 2.65e6 shell-pair
+x centre tasks; gradients with respect to every atom (the shell centres and the GRPP centre of each
+task, derivative shells up to l = 9).
+
+Matrices.  The four groups of eight centres are built by one plan each (about 50 GB of device
+memory per group) and their matrices summed: the sum over centres is the reference driver's
+(eval_integrals.c:42-130).  Checked at full size: finite, bitwise repeatable, and 32 random ordered
+pairs with l_A + l_B <= 4 x all 32 centres against the genuine reference (rule or 3 floors).  At
+this class's 3-bohr separations the l = 8 + 8 type-2 blocks cancel terms of ~1e16 in libgrpp's
+formulation (every shell expanded about the GRPP centre): their values (up to ~1e19 here) are
+rounding noise in the reference and in this engine alike, so they are not compared.
+
+Gradients.  A gradient plan's blocks are scattered into per-atom gradient matrices on the host;
+libgrpp_batch_plan_download_gradients_atoms streams the atoms one wants (12 dim^2 doubles each)
+instead of all 12 x 32 x dim^2 (85 GB).
+  * The type-1 (local) operator, whose matrices are smooth in the geometry: the local potentials of
+    the first eight centres, every shell; the gradient with respect to atom 0 against a five-point
+    finite difference of the engine's own matrices.
+  * The whole operator of centre 0 (its GRPP only, every shell, every atom differentiated): finite,
+    bitwise repeatable, and the sampled tasks' gradient blocks (l_A + l_B <= 3, shells on the first
+    four atoms, with respect to the atoms of A and B and the centre) against the reference's
+    libgrpp_full_grpp_integrals_gradient."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, FLOOR_FACTOR, error_summary, within_floor
+from libgrpp_amd import configs, io
+
+pytestmark = pytest.mark.gpu
+OPS = ["arep", "so_x", "so_y", "so_z"]
+GROUP = 8   # centres per matrix plan
+
+
+@pytest.fixture(scope="module")
+def c4():
+    return configs.high_l_cluster()
+
+
+def _atom_of(sysm):
+    atoms = [c for c, _ in sysm.centers]
+    return [atoms.index(s.origin) for s in sysm.shells]
+
+
+def _moved(sysm, atom, k, h):
+    """the system with atom `atom` (its GRPP centre and the shells sitting on it) moved by h along k"""
+    c0 = sysm.centers[atom][0]
+    shift = lambda p: tuple(p[i] + (h if i == k else 0.0) for i in range(3))
+    shells = [io.Shell(s.L, shift(s.origin) if s.origin == c0 else s.origin, s.coeffs, s.alpha) for s in sysm.shells]
+    centers = [(shift(p) if a == atom else p, g) for a, (p, g) in enumerate(sysm.centers)]
+    return io.System(shells, centers)
+
+
+def test_high_l_cluster_matrices_full_size(c4):
+    from libgrpp_amd import lib as G
+    assert len(c4.shells) == 288 and len(c4.centers) == 32 and c4.dim == 5280
+    total = [np.zeros((c4.dim, c4.dim)) for _ in range(4)]
+    ntask = 0
+    for g0 in range(0, len(c4.centers), GROUP):
+        plan = G.Plan(io.System(c4.shells, c4.centers[g0:g0 + GROUP]))
+        plan.execute()
+        M = plan.download()
+        st = plan.stats()
+        if g0 == 0:   # repeatability at full size
+            plan.execute()
+            again = plan.download()
+            for k in range(4):
+                assert np.array_equal(M[k], again[k])
+            del again
+        plan.close()
+        ntask += st["n_tasks"]
+        print(f"centres {g0}-{g0 + GROUP - 1}: {st['n_tasks']} tasks, {st['n_radial_entries']} radial entries, device "
+              f"{st['device_seconds']:.3f} s", flush=True)
+        for k in range(4):
+            assert np.isfinite(M[k]).all()
+            total[k] += M[k]
+        del M
+    assert ntask == configs.n_tasks(c4) == 288 * 288 * 32
+    d = np.load(os.path.join(GOLDEN, "high_l_cluster_reference.npz"))
+    off = c4.offsets() + [c4.dim]
+    ns = len(c4.shells)
+    worst = {nm: [0.0, 0.0, 0.0] for nm in OPS}
+    bad = []
+    for b, p in enumerate(d["pair"]):
+        i, j = divmod(int(p), ns)
+        assert c4.shells[i].L + c4.shells[j].L <= 4
+        o0, o1 = d["offset"][b], d["offset"][b + 1]
+        n = (o1 - o0) // 4
+        for k, nm in enumerate(OPS):
+            got = total[k][off[i]:off[i + 1], off[j]:off[j + 1]].ravel()
+            want, fl = d["ref"][o0 + k * n:o0 + (k + 1) * n], float(d["floor"][b][k])
+            a, r, q = error_summary(got, want, fl)
+            worst[nm] = [max(worst[nm][0], a), max(worst[nm][1], r), max(worst[nm][2], q)]
+            if not within_floor(got, want, fl).all():
+                bad.append((int(p), nm, a, fl))
+    for nm in OPS:
+        print(f"high_l_cluster {nm}: max abs {worst[nm][0]:.3e}, max rel {worst[nm][1]:.3e}, worst error/floor "
+              f"{worst[nm][2]:.2f} ({len(d['pair'])} pairs x 32 centres)", flush=True)
+    assert not bad, f"{len(bad)} block operators beyond {FLOOR_FACTOR} floors, first {bad[:3]}"
+
+
+def test_high_l_cluster_local_gradients_full_size(c4):
+    """type-1 operator of the first eight centres' local potentials, all 288 shells (derivative
+    shells up to l = 9): d/dx of atom 0 against [8 (M(+h) - M(-h)) - (M(+2h) - M(-2h))] / 12h"""
+    from libgrpp_amd import lib as G
+    sysm = io.System(c4.shells, [(p, io.GRPP(local=g.local) if a < GROUP else None)
+                                 for a, (p, g) in enumerate(c4.centers)])
+    plan = G.GradientPlan(sysm)
+    plan.execute()
+    an = plan.download_atoms([0])[0, 0]   # arep, d/dx of atom 0
+    plan.close()
+    h = 5e-4
+    M = {}
+    for s in (-2, -1, 1, 2):
+        mv = _moved(sysm, 0, 0, s * h)
+        M[s] = G.batch_matrices(io.System(mv.shells, mv.centers[:GROUP]))[0]
+    fd = (8.0 * (M[1] - M[-1]) - (M[2] - M[-2])) / (12.0 * h)
+    scale, dev = np.abs(an).max(), np.abs(fd - an).max()
+    print(f"local gradients, atom 0 x: max |FD - analytic| {dev:.3e}, largest element {scale:.3e}", flush=True)
+    assert np.isfinite(an).all() and scale > 0.0
+    assert dev <= 1e-6 * scale
+
+
+def test_high_l_cluster_centre_gradients_full_size(c4):
+    """centre 0's whole GRPP (semilocal, spin-orbit, outercore, local) over all 288 shells, every
+    atom differentiated: finite, bitwise repeatable, and the reference's per-task gradients of
+    sampled pairs (with respect to the atoms of A and B and the centre) under the rule or 3 floors"""
+    from libgrpp_amd import lib as G
+    d = np.load(os.path.join(GOLDEN, "high_l_cluster_gradient_reference.npz"))
+    tasks = d["tasks"]
+    atoms = sorted({int(t[2]) for t in tasks})
+    slot = {a: s for s, a in enumerate(atoms)}
+    sysm = io.System(c4.shells, [(p, g if a == 0 else None) for a, (p, g) in enumerate(c4.centers)])
+    plan = G.GradientPlan(sysm)
+    plan.execute()
+    Gr = plan.download_atoms(atoms)
+    plan.execute()
+    again = plan.download_atoms([atoms[-1]])
+    st = plan.stats()
+    plan.close()
+    print(f"centre 0 gradients: {st['n_radial_entries']} radial entries, device {st['device_seconds']:.3f} s, "
+          f"atoms {atoms}", flush=True)
+    assert np.isfinite(Gr).all()
+    assert np.array_equal(Gr[:, 3 * len(atoms) - 3:], again)
+    del again
+    off = c4.offsets() + [c4.dim]
+    worst = {nm: [0.0, 0.0, 0.0] for nm in OPS}
+    bad = []
+    for b, (a, bb, at) in enumerate(tasks):
+        assert c4.shells[a].L + c4.shells[bb].L <= 3
+        o0, o1 = d["offset"][b], d["offset"][b + 1]
+        na, nb = c4.shells[a].cart_size, c4.shells[bb].cart_size
+        want = d["ref"][o0:o1].reshape(4, 3, na, nb)
+        for op, nm in enumerate(OPS):
+            got = Gr[op, 3 * slot[int(at)]:3 * slot[int(at)] + 3, off[a]:off[a + 1], off[bb]:off[bb + 1]]
+            fl = float(d["floor"][b][op])
+            e, r, q = error_summary(got.ravel(), want[op].ravel(), fl)
+            worst[nm] = [max(worst[nm][0], e), max(worst[nm][1], r), max(worst[nm][2], q)]
+            if not within_floor(got.ravel(), want[op].ravel(), fl).all():
+                bad.append(((int(a), int(bb), int(at)), nm, e, fl))
+    for nm in OPS:
+        print(f"high_l_cluster gradients {nm}: max abs {worst[nm][0]:.3e}, max rel {worst[nm][1]:.3e}, worst "
+              f"error/floor {worst[nm][2]:.2f} ({len(tasks)} tasks)", flush=True)
+    assert not bad, f"{len(bad)} task operators beyond {FLOOR_FACTOR} floors, first {bad[:3]}"
