@@ -502,14 +502,22 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
                 // contiguous over (m, k2), as row j of the ket projectors is, so each output is one
                 // dot product of length nm * KB (the same terms in the same order as a loop over m
                 // and k2)
-                const int nmk = nm * KB;
-                for (int e = tid; e < nr * nmk; e += blockDim.x) {
-                    const int im = e / KB, k2 = e - im * KB;   // im = i * nm + m
+                const int nmk = nm * KB, KB2 = (KB + 1) >> 1;
+                // two neighbouring k2 per thread share each P^A load (each sum still runs over k1 in order)
+                for (int e = tid; e < nr * nm * KB2; e += blockDim.x) {
+                    const int im = e / KB2, k2 = 2 * (e - im * KB2);   // im = i * nm + m
+                    const bool two = k2 + 1 < KB;
+                    const int k2b = two ? k2 + 1 : k2;
                     const double *pa = PA + ((long long) r0 * nm + im) * KA;
-                    double sacc = 0.0;
+                    double s0 = 0.0, s1 = 0.0;
 #pragma unroll 4
-                    for (int k1 = 0; k1 < KA; k1++) sacc += pa[k1] * Qs[k1 * KB + k2];
-                    Ts[e] = sacc;
+                    for (int k1 = 0; k1 < KA; k1++) {
+                        const double a = pa[k1];
+                        s0 += a * Qs[k1 * KB + k2];
+                        s1 += a * Qs[k1 * KB + k2b];
+                    }
+                    Ts[im * KB + k2] = s0;
+                    if (two) Ts[im * KB + k2 + 1] = s1;
                 }
                 __syncthreads();
 #pragma unroll
@@ -525,14 +533,20 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
                         for (int x = 0; x < nmk; x++) sacc += tr[x] * pb[x];
                         acc[0][q] += fac * sacc;
                     } else {
-#pragma unroll
-                        for (int eta = 0; eta < 3; eta++) {
-                            const double *pl = PLB + ((long long) eta * ncB + j) * nmk;
-                            double sacc = 0.0;
-#pragma unroll 8
-                            for (int x = 0; x < nmk; x++) sacc += tr[x] * pl[x];
-                            acc[1 + eta][q] += fac * sacc;
+                        // the three L-products side by side (each sum in its own order over x)
+                        const double *pl0 = PLB + (long long) j * nmk, *pl1 = pl0 + (long long) ncB * nmk,
+                                     *pl2 = pl1 + (long long) ncB * nmk;
+                        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll 4
+                        for (int x = 0; x < nmk; x++) {
+                            const double t = tr[x];
+                            s0 += t * pl0[x];
+                            s1 += t * pl1[x];
+                            s2 += t * pl2[x];
                         }
+                        acc[1][q] += fac * s0;
+                        acc[2][q] += fac * s1;
+                        acc[3][q] += fac * s2;
                     }
                 }
                 __syncthreads();

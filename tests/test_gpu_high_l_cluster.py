@@ -7,7 +7,8 @@ task, derivative shells up to l = 9).
 
 Matrices.  The four groups of eight centres are built by one plan each (about 50 GB of device
 memory per group) and their matrices summed: the sum over centres is the reference driver's
-(eval_integrals.c:42-130).  Checked at full size: finite, bitwise repeatable, and 32 random ordered
+(eval_integrals.c:42-130).  Checked at full size: finite, bitwise repeatable (also replayed as one
+hipGraph with the radial rows in an arbitrary order), and 32 random ordered
 pairs with l_A + l_B <= 4 x all 32 centres against the genuine reference (rule or 3 floors).  At
 this class's 3-bohr separations the l = 8 + 8 type-2 blocks cancel terms of ~1e16 in libgrpp's
 formulation (every shell expanded about the GRPP centre): their values (up to ~1e19 here) are
@@ -41,11 +42,6 @@ def c4():
     return configs.high_l_cluster()
 
 
-def _atom_of(sysm):
-    atoms = [c for c, _ in sysm.centers]
-    return [atoms.index(s.origin) for s in sysm.shells]
-
-
 def _moved(sysm, atom, k, h):
     """the system with atom `atom` (its GRPP centre and the shells sitting on it) moved by h along k"""
     c0 = sysm.centers[atom][0]
@@ -72,6 +68,19 @@ def test_high_l_cluster_matrices_full_size(c4):
                 assert np.array_equal(M[k], again[k])
             del again
         plan.close()
+        if g0 == 0:
+            # the same group captured as one hipGraph, radial rows in an arbitrary order (the level
+            # predictor's hook): replays bitwise equal to the eager build
+            gplan = G.Plan(io.System(c4.shells, c4.centers[g0:g0 + GROUP]), G.BATCH_GRAPH)
+            n = gplan.features().shape[0]
+            gplan.set_predictions(np.random.default_rng(11).random((n, 3)).astype(np.float32))
+            for _ in range(2):
+                gplan.execute()
+                rep = gplan.download()
+                for k in range(4):
+                    assert np.array_equal(M[k], rep[k])
+            del rep
+            gplan.close()
         ntask += st["n_tasks"]
         print(f"centres {g0}-{g0 + GROUP - 1}: {st['n_tasks']} tasks, {st['n_radial_entries']} radial entries, device "
               f"{st['device_seconds']:.3f} s", flush=True)
@@ -122,7 +131,7 @@ def test_high_l_cluster_local_gradients_full_size(c4):
     scale, dev = np.abs(an).max(), np.abs(fd - an).max()
     print(f"local gradients, atom 0 x: max |FD - analytic| {dev:.3e}, largest element {scale:.3e}", flush=True)
     assert np.isfinite(an).all() and scale > 0.0
-    assert dev <= 1e-6 * scale
+    assert dev <= 1e-8 * scale   # measured 1.9e-10 of the scale (h = 5e-4)
 
 
 def test_high_l_cluster_centre_gradients_full_size(c4):
