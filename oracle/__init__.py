@@ -20,7 +20,6 @@ from libgrpp_amd.io import System
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORC_SO = os.path.join(HERE, "liborc.so")
-ORC_NOFMA_SO = os.path.join(HERE, "liborc_nofma.so")
 REF_SO = os.path.join(HERE, "_ref", "libgrpp_ref.so")
 REF_NOFMA_SO = os.path.join(HERE, "_ref", "libgrpp_ref_nofma.so")
 REF_ASSOC_SO = os.path.join(HERE, "_ref", "libgrpp_ref_assoc.so")
@@ -109,18 +108,6 @@ class CpuLib:
 
 
 _cache = {}
-
-
-def oracle_nofma() -> CpuLib:
-    """The oracle compiled without floating-point contraction and with the quadrature points summed
-    in reverse order: the same algorithm under a second, equally valid rounding.  Its spread against
-    ``oracle()`` measures how far the reference formulation is determined by its own arithmetic on a
-    given input."""
-    if "nofma" not in _cache:
-        if not os.path.exists(ORC_NOFMA_SO):
-            raise RuntimeError("oracle/liborc_nofma.so is not built (make -C oracle)")
-        _cache["nofma"] = CpuLib(ORC_NOFMA_SO, "oracle")
-    return _cache["nofma"]
 
 
 def oracle() -> CpuLib:

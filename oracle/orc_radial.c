@@ -139,17 +139,11 @@ static int screen_contracted(int l1, int l2, int n, double CA2, double CB2, cons
     return 0;
 }
 
-/* sum of w U F1 F2 r^n over the grid points [b, e), added to s.  ORC_ALT_ORDER (the second oracle
- * build) takes the points in reverse order and the factors in reverse association: another valid
- * rounding of the same quadrature, used only to measure how strongly an input amplifies rounding */
+/* sum of w U F1 F2 r^n over the grid points [b, e), added to s (radial_type2_integral.c:285, :304) */
 static inline double point_sum(const double *w, const double *U, const double *F1, const double *F2,
                                const double *rN, int b, int e, double s)
 {
-#ifdef ORC_ALT_ORDER
-    for (int i = e - 1; i >= b; i--) s += rN[i] * (F2[i] * (F1[i] * (U[i] * w[i])));
-#else
     for (int i = b; i < e; i++) s += w[i] * U[i] * F1[i] * F2[i] * rN[i];
-#endif
     return s;
 }
 
