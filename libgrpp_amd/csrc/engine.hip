@@ -639,6 +639,62 @@ __device__ void mmd_accumulate(const MmdShell A, const MmdShell B, double Cx, do
     const int ssz = (N + 1) * (N + 2) * (N + 3) / 6;  // simplex layer
     __shared__ double w_s[64], q_s[64];
     __shared__ int n_s;
+    const int npair = A.nprim * B.nprim;
+    {
+        // r^0 terms (n_k = 2, type1_mcmurchie_davidson.c:397-473) whose t = 0 E columns of every
+        // (term, primitive pair) fit the scratch together: all tables in one pass and one barrier,
+        // then every output sums them term by term and, within a term, in the same batches of
+        // pairs as the loop below (bitwise the same sums, two barriers instead of two per term)
+        bool all2 = true;
+        for (int tk = t0; tk < t1; tk++) all2 = all2 && (tpow[tk] == 2);
+        const int nunit = (t1 - t0) * npair, bmax = min(64, scratch_len / e0sz);
+        if (all2 && nunit > 0 && nunit * (e0sz + 1) <= scratch_len) {
+            double *wbuf = scratch + (size_t) nunit * e0sz;
+            for (int e = tid; e < nunit * 3; e += nt) {
+                const int u = e / 3, dir = e - 3 * u, tk = t0 + u / npair, pp = u - (u / npair) * npair;
+                const int ia = pp / B.nprim, ib = pp % B.nprim;
+                const double a = alpha[A.poff + ia], b = alpha[B.poff + ib], c = tzeta[tk];
+                const double q = a + b + c;
+                const double Ad = dir == 0 ? A.x : (dir == 1 ? A.y : A.z);
+                const double Bd = dir == 0 ? B.x : (dir == 1 ? B.y : B.z);
+                const double Cd = dir == 0 ? Cx : (dir == 1 ? Cy : Cz);
+                const double Qd = (a * Ad + b * Bd + c * Cd) / q;
+                const double XAB = Ad - Bd, XAC = Ad - Cd, XBC = Bd - Cd;
+                const double K = exp(-(a * b / q) * XAB * XAB) * exp(-(a * c / q) * XAC * XAC) * exp(-(b * c / q) * XBC * XBC);
+                mmd_E0_dir(scratch + (size_t) u * e0sz + dir * (la + 1) * (lb + 1), la, lb, q, K, Qd - Ad, Qd - Bd);
+                if (dir == 0) {
+                    const double ca = craw[A.poff + ia], cb = craw[B.poff + ib];
+                    const double skip = (fabs(ca) < 1e-14 || fabs(cb) < 1e-14) ? 0.0 : 1.0;
+                    const double NA = gnorm(la, a), NB = gnorm(lb, b);
+                    wbuf[u] = skip * tcoef[tk] * ca * cb * (NA * NB * pow(M_PI / q, 1.5));
+                }
+            }
+            __syncthreads();
+            for (int e = tid; e < nel; e += nt) {
+                const int i = e / ncB, j = e % ncB;
+                int ax, ay, az, bx, by, bz;
+                dev_cart(la, i, ax, ay, az);
+                dev_cart(lb, j, bx, by, bz);
+                const int ox = ax * (lb + 1) + bx, oy = (la + 1) * (lb + 1) + ay * (lb + 1) + by,
+                          oz = 2 * (la + 1) * (lb + 1) + az * (lb + 1) + bz;
+                double a_e = acc[e];
+                for (int tk = 0; tk < t1 - t0; tk++)
+                    for (int p0 = 0; p0 < npair; p0 += bmax) {
+                        const int nb = min(bmax, npair - p0);
+                        double sacc = 0.0;
+                        for (int pb = 0; pb < nb; pb++) {
+                            const int u = tk * npair + p0 + pb;
+                            const double *E = scratch + (size_t) u * e0sz;
+                            sacc += wbuf[u] * (E[ox] * E[oy] * E[oz]);
+                        }
+                        a_e += sacc;
+                    }
+                acc[e] = a_e;
+            }
+            __syncthreads();
+            return;
+        }
+    }
     for (int tk = t0; tk < t1; tk++) {
         const int npow = tpow[tk];
         const double dk = tcoef[tk], zk = tzeta[tk];
@@ -646,7 +702,6 @@ __device__ void mmd_accumulate(const MmdShell A, const MmdShell B, double Cx, do
         int bmax = scratch_len / per;
         if (bmax > 64) bmax = 64;
         if (npow != 2) bmax = 1;
-        const int npair = A.nprim * B.nprim;
         for (int p0 = 0; p0 < npair; p0 += bmax) {
             const int nb = min(bmax, npair - p0);
             // phase 1: E tables (one thread per (pair, direction))

@@ -510,7 +510,7 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
                     const int k2b = two ? k2 + 1 : k2;
                     const double *pa = PA + ((long long) r0 * nm + im) * KA;
                     double s0 = 0.0, s1 = 0.0;
-#pragma unroll 4
+#pragma unroll 8
                     for (int k1 = 0; k1 < KA; k1++) {
                         const double a = pa[k1];
                         s0 += a * Qs[k1 * KB + k2];
@@ -529,7 +529,7 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
                     if (!is_so) {
                         const double *pb = PB + (long long) j * nmk;
                         double sacc = 0.0;
-#pragma unroll 8
+#pragma unroll 16
                         for (int x = 0; x < nmk; x++) sacc += tr[x] * pb[x];
                         acc[0][q] += fac * sacc;
                     } else {
@@ -537,7 +537,7 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
                         const double *pl0 = PLB + (long long) j * nmk, *pl1 = pl0 + (long long) ncB * nmk,
                                      *pl2 = pl1 + (long long) ncB * nmk;
                         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-#pragma unroll 4
+#pragma unroll 8
                         for (int x = 0; x < nmk; x++) {
                             const double t = tr[x];
                             s0 += t * pl0[x];
