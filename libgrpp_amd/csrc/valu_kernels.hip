@@ -461,7 +461,8 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
                 if (!is_so && !(parts & 2)) continue;
                 const int L = p.L, nm = 2 * L + 1;
                 const int KA = dev_kcols(L, la), KB = dev_kcols(L, lb);
-                // radial block Q[(lam1,nA)][(lam2,nB)]
+                // radial block Q[(lam1,nA)][(lam2,nB)]; nz: this thread loaded a nonzero entry
+                bool nz = false;
                 if (onec) {
                     // analytic one-centre branch (radial_type2_integral.c:126-155): only lambda1 = lambda2 = 0
                     for (int e = tid; e < KA * KB; e += blockDim.x) Qs[e] = 0.0;
@@ -483,6 +484,7 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
                             }
                             if (fabs(v) < p.thr) v = 0.0;
                             Qs[ka * KB + kb] = v;
+                            nz = v != 0.0;
                         }
                     }
                 } else {
@@ -490,10 +492,13 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
                     const int rb = J.slot_rowoff[(long long) p.slot * J.nshell + it.b];
                     for (int e = tid; e < KA * KB; e += blockDim.x) {
                         const int k1 = e / KB, k2 = e % KB;
-                        Qs[e] = J.Q[p.qoff + (long long) (ra + k1) * p.M + rb + k2];   // logical order
+                        const double v = J.Q[p.qoff + (long long) (ra + k1) * p.M + rb + k2];   // logical order
+                        Qs[e] = v;
+                        nz = nz || v != 0.0;
                     }
                 }
-                __syncthreads();
+                // a block of zeros adds exactly zero to every output (acc is never -0): skip the channel
+                if (!__syncthreads_or(nz)) continue;
                 const double *PA = J.P + (is_so ? J.slot_pb3 : J.slot_pb2)[(long long) p.slot * J.nshell + it.a];
                 const double *PB = J.P + J.slot_poff[(long long) p.slot * J.nshell + it.b];
                 const double *PLB = is_so ? J.P + J.slot_ploff[(long long) p.slot * J.nshell + it.b] : nullptr;

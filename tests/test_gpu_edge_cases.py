@@ -107,3 +107,23 @@ def test_molecule_without_centres_in_a_molecules_plan(G):
         alone = G.batch_matrices(mols[i])
         for k in range(4):
             assert np.abs(out[i][k] - alone[k]).max() <= 1e-12 * max(1.0, np.abs(alone[k]).max())
+
+
+def test_concurrent_per_call_integrators_match_sequential(G):
+    """host threads calling libgrpp_full_grpp_integrals at once (the reference driver's OpenMP
+    pattern, eval_integrals.c:78-123) on shell pairs of very different LDS needs (l = 0..6 against a
+    GRPP with L <= 5, spin-orbit and outercore s/p/d): every result bitwise equal to the same call made
+    alone; the per-kernel dynamic-LDS limits are set once per device, not per call"""
+    from concurrent.futures import ThreadPoolExecutor
+    s = configs.high_l_cluster(seed=1, natom=2, nprim=3, alpha=(0.5, 20.0), lmax_shell=6)
+    C, g = s.centers[0]
+    pairs = [(a, b) for a in range(len(s.shells)) for b in range(len(s.shells)) if (a + 3 * b) % 5 == 0]
+    api = G.Api()
+    seq = [api.full(s.shells[a], s.shells[b], C, g) for a, b in pairs]
+    jobs = [k for k in range(len(pairs)) for _ in range(3)]
+    with ThreadPoolExecutor(16) as ex:
+        res = list(ex.map(lambda k: (k, api.full(s.shells[pairs[k][0]], s.shells[pairs[k][1]], C, g)), jobs))
+    print(f"{len(jobs)} concurrent calls over {len(pairs)} pairs, l up to {max(x.L for x in s.shells)}", flush=True)
+    for k, got in res:
+        for op in range(4):
+            assert np.array_equal(got[op], seq[k][op]), (pairs[k], op)
