@@ -265,7 +265,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
     // estimate -- inside the table, upper triangle, canonical (lambda1, lambda2, N) split -- with the
     // full-cost ones first and the ones that vanish analytically (K_l(0) = 0 for l >= 1 on a shell
     // sitting on the centre) last, so lanes run entries of like cost side by side.
-    __shared__ int list[RT * RT / SCR_SPLIT];
+    __shared__ unsigned short list[RT * RT / SCR_SPLIT];   // positions < RT * RT / 4 <= 1024
     // per tile position e of the wave: 0 nothing to store (outside the table, lower triangle or not
     // a canonical split), 1 estimate pending, 2 vanishes analytically, 3 a triple failed (NaN),
     // 4 evaluated and stored
@@ -360,19 +360,58 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
         }
     }
     __syncthreads();
-    // the entries the probe did not settle: the whole wave per entry
+    // the entries the probe did not settle: the whole wave per entry.  Their shells, (lambda1,
+    // lambda2, N) and distances to the centre are gathered 64 at a time, one entry per lane (the
+    // loads of all 64 in flight at once), into LDS, and each entry's evaluation reads them from there
+    __shared__ int s_apoff[64], s_bpoff[64], s_np[64], s_lln[64], s_e[64];
+    __shared__ double s_ca[64], s_cb[64];
+    auto rfl = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
+    auto rfl_d = [](double v) {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_readfirstlane((int) (b & 0xffffffffLL));
+        const int hi = __builtin_amdgcn_readfirstlane((int) (b >> 32));
+        return __longlong_as_double(((long long) hi << 32) | (unsigned int) lo);
+    };
 #pragma unroll 1
-    for (int q = 0; q < nf; q++) {
-        const int e = list[q];
-        if (st[e] == 3) continue;
-        int row, col;
-        RowDesc ra, rb;
-        DShell A, B;
-        double CA, CB;
-        entry(q, row, col, ra, rb, A, B, CA, CB);
-        const double v = screen_entry_wave(J, T, A, B, CA, CB, P, ra.lam, rb.lam, ra.n + rb.n, ntrip, nstep);
-        if (threadIdx.x == 0)
-            scr[jb.qoff + (long long) (t.ti * RT + (e / RT) * SCR_SPLIT + sub) * jb.M + t.tj * RT + e % RT] = v;
+    for (int c0 = 0; c0 < nf; c0 += 64) {
+        const int qi = c0 + threadIdx.x;
+        bool live = false;
+        if (qi < nf) {
+            const int e = list[qi];
+            if (st[e] != 3) {
+                int row, col;
+                RowDesc ra, rb;
+                DShell A, B;
+                double CA, CB;
+                entry(qi, row, col, ra, rb, A, B, CA, CB);
+                s_apoff[threadIdx.x] = A.poff;
+                s_bpoff[threadIdx.x] = B.poff;
+                s_np[threadIdx.x] = A.nprim | (B.nprim << 16);
+                s_lln[threadIdx.x] = ra.lam | (rb.lam << 8) | ((ra.n + rb.n) << 16);
+                s_e[threadIdx.x] = e;
+                s_ca[threadIdx.x] = CA;
+                s_cb[threadIdx.x] = CB;
+                live = true;
+            }
+        }
+        unsigned long long m = __ballot(live);
+        __syncthreads();
+#pragma unroll 1
+        while (m) {
+            const int ln = __ffsll((long long) m) - 1;
+            m &= m - 1;
+            DShell A{}, B{};
+            const int np = rfl(s_np[ln]), lln = rfl(s_lln[ln]), e = rfl(s_e[ln]);
+            A.poff = rfl(s_apoff[ln]);
+            A.nprim = np & 0xffff;
+            B.poff = rfl(s_bpoff[ln]);
+            B.nprim = np >> 16;
+            const double v = screen_entry_wave(J, T, A, B, rfl_d(s_ca[ln]), rfl_d(s_cb[ln]), P, lln & 0xff,
+                                               (lln >> 8) & 0xff, lln >> 16, ntrip, nstep);
+            if (threadIdx.x == 0)
+                scr[jb.qoff + (long long) (t.ti * RT + (e / RT) * SCR_SPLIT + sub) * jb.M + t.tj * RT + e % RT] = v;
+        }
+        __syncthreads();   // the next chunk rewrites the entry data
     }
     __syncthreads();
     // the failed (NaN) and vanishing (0) estimates in tile order: a row's 64 columns are 64
