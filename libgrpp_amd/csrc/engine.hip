@@ -1205,6 +1205,9 @@ extern "C" int eng_device_ok(void)
 }
 
 // ================================================================== plan
+// small parts (<= 64 Cartesian pairs) go to the wave-per-part contraction only when there are this many
+constexpr int CTW_MIN_PARTS = 262144;
+
 struct eng_plan {
     int nshell = 0, ncenter = 0, parts = 0, flags = 0;
     long long out_len = 0, op_stride = 0;
@@ -1243,6 +1246,7 @@ struct eng_plan {
     double *d_partial = nullptr;     // per-part type-1 blocks (k_type1)
     double *d_partial_ct = nullptr;  // per-part semilocal, spin-orbit and outercore blocks (k_contract)
     int n_parts = 0;
+    int n_parts_small = 0;           // parts of <= 64 Cartesian pairs: the tail of d_parts_run (k_contract_w)
     unsigned long long *d_counters = nullptr;
     eng_tuning_t tune{1, 128, 256, 0, 4};   // launch shapes and stream layout; none of them changes a result
     hipEvent_t ev[8] = {};   // execution phases on the caller's stream: see eng_plan_execute   // execution phases on the caller's stream: see eng_plan_execute
@@ -1266,6 +1270,7 @@ struct eng_plan {
     int radial_mfma = 0;
     int nm_max = 1;
     int ct_qs = 1, ct_ts = 1;   // k_contract LDS doubles
+    int ctw_qs = 64, ctw_ts = 64;   // k_contract_w LDS doubles per wave
     int t1_scratch = 64, t1_acc = 1;   // k_type1 LDS doubles
     int ocv_scratch = 64, ocv_nel = 1;  // k_ocv LDS doubles: E / R tables, each of its two blocks
     // radial rows: logical order per slot (shell, n, lambda), their physical order in the tables,
@@ -1791,7 +1796,25 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
             const int la = sh[items[q.item].a].L, lb = sh[items[q.item].b].L;
             return (long long) ncart(la) * ncart(lb) * (la + lb + 1);
         };
-        std::stable_sort(run.begin(), run.end(), [&](const PartItem &x, const PartItem &y) { return cost(x) > cost(y); });
+        // parts of at most 64 Cartesian pairs go last, to the wave-per-part contraction (type-1 keeps
+        // the same order)
+        auto small = [&](const PartItem &q) {
+            return ncart(sh[items[q.item].a].L) * ncart(sh[items[q.item].b].L) <= 64;
+        };
+        std::stable_sort(run.begin(), run.end(), [&](const PartItem &x, const PartItem &y) {
+            const bool sx = small(x), sy = small(y);
+            return sx != sy ? sy : cost(x) > cost(y);
+        });
+        P->n_parts_small = (int) std::count_if(run.begin(), run.end(), small);
+        // the separate launch pays a second drain of the chip; below a few hundred thousand small
+        // parts (heavy_cluster: 100,352 parts in all) the single launch is faster (measured)
+        // GRPP_CTW_MIN_PARTS overrides the threshold (tests compare both paths on small inputs)
+        const char *ctw = getenv("GRPP_CTW_MIN_PARTS");
+        const long long ctw_min = ctw ? atoll(ctw) : CTW_MIN_PARTS;
+        if (P->n_parts_small < ctw_min || P->n_parts_small == 0) {
+            std::stable_sort(run.begin(), run.end(), [&](const PartItem &x, const PartItem &y) { return cost(x) > cost(y); });
+            P->n_parts_small = 0;
+        }
         rc |= dput(P, &P->d_parts_run, run);
     }
     rc |= dput(P, &P->d_item_part0, item_part0);
@@ -1884,6 +1907,10 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
         }
         P->ct_qs = kmax * kmax;
         P->ct_ts = std::min(TS_MAX, P->nm_max * ncmax * kmax);
+        // small parts: KA <= ncart(la), KB <= ncart(lb), so a Q block has <= 64 doubles and the whole
+        // intermediate nm x ncA x KB <= nm_max x 64 fits in one pass
+        P->ctw_qs = 64;
+        P->ctw_ts = P->nm_max * 64;
     }
 
     DevJob &J = P->J;
@@ -2026,10 +2053,16 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
     HIPCHK(rec(P->ev[4], st));
     HIPCHK(wait(st, P->evs[1]));   // the contraction reads projectors and outercore vectors
     HIPCHK(rec(P->ev[6], st));
-    if (P->n_parts > 0)
-        launch_contract(P->tune.contract_threads, P->n_parts, (size_t) (P->ct_qs + P->ct_ts) * sizeof(double), st,
+    // the parts of more than 64 Cartesian pairs (largest first), then the small ones four per workgroup
+    const int n_large = P->n_parts - P->n_parts_small;
+    if (n_large > 0)
+        launch_contract(P->tune.contract_threads, n_large, (size_t) (P->ct_qs + P->ct_ts) * sizeof(double), st,
                         P->d_parts_run, P->d_items, P->J, P->T, P->d_partial_ct, P->parts, P->nm_max, P->ct_qs,
                         P->ct_ts);
+    if (P->n_parts_small > 0)
+        launch_contract_w(P->n_parts_small, (size_t) (P->ctw_qs + P->ctw_ts) * sizeof(double), st,
+                          P->d_parts_run + n_large, P->d_items, P->J, P->T, P->d_partial_ct, P->parts, P->nm_max,
+                          P->ctw_qs, P->ctw_ts);
     HIPCHK(wait(st, P->evs[2]));   // join: the reduction reads the type-1 slices
     if (P->n_items > 0)   // also writes the zero blocks of items without centres
         hipLaunchKernelGGL(k_contract_reduce, dim3(P->n_items), dim3(256), 0, st, P->d_items, P->d_item_part0,

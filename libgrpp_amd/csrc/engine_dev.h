@@ -218,5 +218,10 @@ void launch_screen(int probe, int split, int n_tiles, hipStream_t st, const Tile
 void launch_contract(int threads, int n_parts, size_t lds_bytes, hipStream_t st, const PartItem *partlist,
                      const WorkItem *items, const DevJob &J, const DevTables &T, double *partial, int parts,
                      int nm_max, int qs_len, int ts_len);
+// k_contract_w: parts of at most 64 Cartesian pairs, four per workgroup (one per wave), each wave
+// with lds_bytes_per_wave of dynamic LDS (qs_len + ts_len doubles)
+void launch_contract_w(int n_parts, size_t lds_bytes_per_wave, hipStream_t st, const PartItem *partlist,
+                       const WorkItem *items, const DevJob &J, const DevTables &T, double *partial, int parts,
+                       int nm_max, int qs_len, int ts_len);
 // the k_contract instance of a workgroup size (kernel attributes)
 const void *contract_kernel_ptr(int threads);

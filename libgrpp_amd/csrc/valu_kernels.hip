@@ -455,25 +455,39 @@ __device__ __forceinline__ double int_gauss(int n, double a)
 }
 
 
-// One workgroup per (shell pair, centre) (PartItem): the centre's contribution to the four blocks
-// goes to its own slice of the contraction's partial buffer; k_contract_reduce adds, in centre
-// order, each centre's type-1 slice plus its contraction slice.
-template <int CT_THREADS>
-__global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_contract(const PartItem *__restrict__ partlist, const WorkItem *__restrict__ items,
-                                                  DevJob J, DevTables T, double *__restrict__ partial, int parts,
-                                                  int nm_max, int qs_len, int ts_len)
+// LDS visibility among the lanes of one wave (the wave-per-part contraction has no block barrier)
+__device__ __forceinline__ void wave_sync()
 {
-    constexpr int CT_EPT = 256 / CT_THREADS;
-    // dynamic LDS sized by the plan: Q block (qs_len doubles) + (i, m, k2) intermediate (ts_len)
-    extern __shared__ double ct_lds[];
-    double *Qs = ct_lds, *Ts = ct_lds + qs_len;
-    const PartItem pt = partlist[blockIdx.x];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The contraction of one (shell pair, centre) part by CT_THREADS threads (tid = 0 .. CT_THREADS-1)
+// with up to CT_EPT outputs each per pass, the Q block and the intermediate in Qs / Ts; WAVE: the
+// threads are one wave of a workgroup whose other waves work on other parts, so synchronisation is
+// per wave
+template <int CT_THREADS, int CT_EPT, bool WAVE>
+__device__ __forceinline__ void contract_part(const PartItem pt, const WorkItem *__restrict__ items, const DevJob &J,
+                                              const DevTables &T, double *__restrict__ partial, int parts, int nm_max,
+                                              int ts_len, double *Qs, double *Ts, const int tid)
+{
+    auto sync = [] {
+        if (WAVE) wave_sync();
+        else __syncthreads();
+    };
+    auto sync_or = [](bool v) -> bool {
+        if (WAVE) {
+            wave_sync();
+            return __ballot(v) != 0;
+        }
+        return __syncthreads_or(v);
+    };
     WorkItem it = items[pt.item];
     it.cb = pt.ci;
     it.ce = pt.ci + 1;
     const DShell sa = J.sh[it.a], sb = J.sh[it.b];
     const int la = sa.L, lb = sb.L, ncA = dev_ncart(la), ncB = dev_ncart(lb);
-    const int tid = threadIdx.x;
     const double FOURPI2 = 16.0 * M_PI * M_PI;
     // rows of A per pass: the outputs fit CT_EPT per thread and T[i][m][k2] fits TS_MAX
     const int rows_per_chunk = max(1, min((CT_EPT * CT_THREADS) / ncB, ts_len / (nm_max * ncB)));
@@ -504,8 +518,8 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
                 bool nz = false;
                 if (onec) {
                     // analytic one-centre branch (radial_type2_integral.c:126-155): only lambda1 = lambda2 = 0
-                    for (int e = tid; e < KA * KB; e += blockDim.x) Qs[e] = 0.0;
-                    __syncthreads();
+                    for (int e = tid; e < KA * KB; e += CT_THREADS) Qs[e] = 0.0;
+                    sync();
                     if (tid < (la + 1) * (lb + 1)) {
                         const int nA = tid / (lb + 1), nB = tid % (lb + 1);
                         if (dev_lam_lo(L, nA) == 0 && dev_lam_lo(L, nB) == 0) {
@@ -529,7 +543,7 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
                 } else {
                     const int ra = J.slot_rowoff[(long long) p.slot * J.nshell + it.a];
                     const int rb = J.slot_rowoff[(long long) p.slot * J.nshell + it.b];
-                    for (int e = tid; e < KA * KB; e += blockDim.x) {
+                    for (int e = tid; e < KA * KB; e += CT_THREADS) {
                         const int k1 = e / KB, k2 = e % KB;
                         const double v = J.Q[p.qoff + (long long) (ra + k1) * p.M + rb + k2];   // logical order
                         Qs[e] = v;
@@ -537,7 +551,7 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
                     }
                 }
                 // a block of zeros adds exactly zero to every output (acc is never -0): skip the channel
-                if (!__syncthreads_or(nz)) continue;
+                if (!sync_or(nz)) continue;
                 const double *PA = J.P + (is_so ? J.slot_pb3 : J.slot_pb2)[(long long) p.slot * J.nshell + it.a];
                 const double *PB = J.P + J.slot_poff[(long long) p.slot * J.nshell + it.b];
                 const double *PLB = is_so ? J.P + J.slot_ploff[(long long) p.slot * J.nshell + it.b] : nullptr;
@@ -548,7 +562,7 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
                 // and k2)
                 const int nmk = nm * KB, KB2 = (KB + 1) >> 1;
                 // two neighbouring k2 per thread share each P^A load (each sum still runs over k1 in order)
-                for (int e = tid; e < nr * nm * KB2; e += blockDim.x) {
+                for (int e = tid; e < nr * nm * KB2; e += CT_THREADS) {
                     const int im = e / KB2, k2 = 2 * (e - im * KB2);   // im = i * nm + m
                     const bool two = k2 + 1 < KB;
                     const int k2b = two ? k2 + 1 : k2;
@@ -563,7 +577,7 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
                     Ts[im * KB + k2] = s0;
                     if (two) Ts[im * KB + k2 + 1] = s1;
                 }
-                __syncthreads();
+                sync();
 #pragma unroll
                 for (int q = 0; q < CT_EPT; q++) {
                     const int e = tid + q * CT_THREADS;
@@ -593,7 +607,7 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
                         acc[3][q] += fac * s2;
                     }
                 }
-                __syncthreads();
+                sync();
             }
             // ---- outercore projectors: part 1 (outercore_integrals.c:147-270) and part 2
             // (:277-390); a part-2-only job may restrict part 2 to one (oi, oj) term ----
@@ -671,8 +685,38 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
 #pragma unroll
             for (int op = 0; op < 4; op++) partial[pt.off + ((long long) op * ncA + i) * ncB + j] = acc[op][q];
         }
-        __syncthreads();
+        sync();
     }
+}
+
+// One workgroup per (shell pair, centre) (PartItem): the centre's contribution to the four blocks
+// goes to its own slice of the contraction's partial buffer; k_contract_reduce adds, in centre
+// order, each centre's type-1 slice plus its contraction slice.
+template <int CT_THREADS>
+__global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_contract(const PartItem *__restrict__ partlist, const WorkItem *__restrict__ items,
+                                                  DevJob J, DevTables T, double *__restrict__ partial, int parts,
+                                                  int nm_max, int qs_len, int ts_len)
+{
+    // dynamic LDS sized by the plan: Q block (qs_len doubles) + (i, m, k2) intermediate (ts_len)
+    extern __shared__ double ct_lds[];
+    contract_part<CT_THREADS, 256 / CT_THREADS, false>(partlist[blockIdx.x], items, J, T, partial, parts, nm_max, ts_len, ct_lds,
+                                     ct_lds + qs_len, threadIdx.x);
+}
+
+// Parts of at most 64 Cartesian pairs (the tail of the launch order): four waves per workgroup,
+// one part per wave, each with its own LDS (qs_len + ts_len doubles); the sums are those of
+// k_contract (same chunking of rows into one pass, same order of every sum)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_contract_w(const PartItem *__restrict__ partlist, int n_parts,
+                                                  const WorkItem *__restrict__ items, DevJob J, DevTables T,
+                                                  double *__restrict__ partial, int parts, int nm_max, int qs_len,
+                                                  int ts_len)
+{
+    extern __shared__ double ct_lds[];
+    const int w = threadIdx.x >> 6, p = blockIdx.x * 4 + w;
+    if (p >= n_parts) return;   // whole waves leave; nothing below synchronises the block
+    double *base = ct_lds + (size_t) w * (qs_len + ts_len);
+    contract_part<64, 1, true>(partlist[p], items, J, T, partial, parts, nm_max, ts_len, base, base + qs_len,
+                            threadIdx.x & 63);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -689,6 +733,14 @@ void launch_screen(int sp, int ss, int n_tiles, hipStream_t st, const TileItem *
                 : ss == 8  ? (sp == 1 ? k_screen<1, 8> : sp == 2 ? k_screen<2, 8> : k_screen<4, 8>)
                            : (sp == 1 ? k_screen<1, 4> : sp == 2 ? k_screen<2, 4> : k_screen<4, 4>);
     hipLaunchKernelGGL(screen, dim3(ss * n_tiles), dim3(64), 0, st, tiles, jobs, rows, J, T, scr, counters);
+}
+
+void launch_contract_w(int n_parts, size_t lds_bytes_per_wave, hipStream_t st, const PartItem *partlist,
+                       const WorkItem *items, const DevJob &J, const DevTables &T, double *partial, int parts,
+                       int nm_max, int qs_len, int ts_len)
+{
+    hipLaunchKernelGGL(k_contract_w, dim3((n_parts + 3) / 4), dim3(256), 4 * lds_bytes_per_wave, st, partlist, n_parts,
+                       items, J, T, partial, parts, nm_max, qs_len, ts_len);
 }
 
 const void *contract_kernel_ptr(int threads)

@@ -127,3 +127,16 @@ def test_concurrent_per_call_integrators_match_sequential(G):
     for k, got in res:
         for op in range(4):
             assert np.array_equal(got[op], seq[k][op]), (pairs[k], op)
+
+
+@pytest.mark.parametrize("workload", ["small_pair", "heavy_cluster"])
+def test_wave_per_part_contraction_equals_block_contraction(G, workload, monkeypatch):
+    """the contraction of parts of <= 64 Cartesian pairs by one wave each (four per workgroup), forced
+    on inputs below its size threshold, gives bitwise the matrices of the block-per-part kernel"""
+    sysm = configs.WORKLOADS[workload]()
+    monkeypatch.setenv("GRPP_CTW_MIN_PARTS", "1000000000")
+    block = G.batch_matrices(sysm)
+    monkeypatch.setenv("GRPP_CTW_MIN_PARTS", "0")
+    wave = G.batch_matrices(sysm)
+    for k in range(4):
+        assert np.array_equal(block[k], wave[k]), k
