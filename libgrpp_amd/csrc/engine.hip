@@ -331,13 +331,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     // the matrix cores work on the current one.
     constexpr int NW = ((LMAX_SHELL + 1) * RK + 255) / 256;
     double fa_n[8], fb_n[8], w_n[NW], rp_n[NW];
+    // staging: thread tid owns row sr = tid / 4 of both operands and the 8 consecutive points
+    // sk0 .. sk0 + 7 of a K step (four 16-byte loads per row, its F row and power n read once per tile)
+    const int sr = tid >> 2, sk0 = (tid & 3) * 8;
+    const int my_fa = sfa[sr], my_fb = sfb[sr], my_na = sna[sr], my_nb = snb[sr];
     auto prefetch = [&](int k0) {
+        const double2 *pa = (const double2 *) (F + (size_t) (my_fa >= 0 ? my_fa : 0) * NPTS + k0 + sk0);
+        const double2 *pb = (const double2 *) (F + (size_t) (my_fb >= 0 ? my_fb : 0) * NPTS + k0 + sk0);
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int e = tid + 256 * q, r = e >> 5, k = k0 + (e & 31);
-            const int fa = sfa[r], fb = sfb[r];
-            fa_n[q] = (fa >= 0) ? F[(size_t) fa * NPTS + k] : 0.0;
-            fb_n[q] = (fb >= 0) ? F[(size_t) fb * NPTS + k] : 0.0;
+        for (int q = 0; q < 4; q++) {
+            const double2 va = pa[q], vb = pb[q];
+            fa_n[2 * q] = (my_fa >= 0) ? va.x : 0.0;
+            fa_n[2 * q + 1] = (my_fa >= 0) ? va.y : 0.0;
+            fb_n[2 * q] = (my_fb >= 0) ? vb.x : 0.0;
+            fb_n[2 * q + 1] = (my_fb >= 0) ? vb.y : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < NW; j++) {
@@ -371,9 +378,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < 8; q++) {
-                const int e = tid + 256 * q, r = e >> 5, kk = e & 31;
-                As[kk][r] = (sfa[r] >= 0) ? fa_n[q] * Rw[sna[r]][kk] : 0.0;
-                Bs[kk][r] = (sfb[r] >= 0) ? fb_n[q] * Rs[snb[r]][kk] : 0.0;
+                As[sk0 + q][sr] = (my_fa >= 0) ? fa_n[q] * Rw[my_na][sk0 + q] : 0.0;
+                Bs[sk0 + q][sr] = (my_fb >= 0) ? fb_n[q] * Rs[my_nb][sk0 + q] : 0.0;
             }
             __syncthreads();
             if (k0 + RK < NPTS) prefetch(k0 + RK);   // in bounds: the last step ends at NPTS
