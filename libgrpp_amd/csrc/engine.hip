@@ -275,8 +275,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
                                                      const int *__restrict__ pos, signed char *__restrict__ lev_out,
                                                      int *__restrict__ tile_lev)
 {
-    __shared__ double As[RK][RT + 2];
-    __shared__ double Bs[RK][RT + 2];
+    __shared__ double As[RK][RT + 1];   // odd row length: the staging stores hit distinct banks
+    __shared__ double Bs[RK][RT + 1];
     __shared__ double Rw[LMAX_SHELL + 1][RK], Rs[LMAX_SHELL + 1][RK];   // r^n sqrt|wU|, r^n sign(wU) sqrt|wU|
     __shared__ int sfa[RT], sna[RT], sfb[RT], snb[RT];
     const TileItem t = tiles[blockIdx.x];
