@@ -1807,9 +1807,19 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
         auto small = [&](const PartItem &q) {
             return ncart(sh[items[q.item].a].L) * ncart(sh[items[q.item].b].L) <= 64;
         };
+        // within a cost, by (centre, A, B): the parts that read the same bra projectors run together
+        auto loc = [&](const PartItem &x, const PartItem &y) {
+            const int cx = cidx[x.ci], cy = cidx[y.ci];
+            if (cx != cy) return cx < cy;
+            const auto &ix = items[x.item];
+            const auto &iy = items[y.item];
+            return ix.a != iy.a ? ix.a < iy.a : ix.b < iy.b;
+        };
         std::stable_sort(run.begin(), run.end(), [&](const PartItem &x, const PartItem &y) {
             const bool sx = small(x), sy = small(y);
-            return sx != sy ? sy : cost(x) > cost(y);
+            if (sx != sy) return sy;
+            const long long cxv = cost(x), cyv = cost(y);
+            return cxv != cyv ? cxv > cyv : loc(x, y);
         });
         P->n_parts_small = (int) std::count_if(run.begin(), run.end(), small);
         // the separate launch pays a second drain of the chip; below a few hundred thousand small
@@ -1818,7 +1828,10 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
         const char *ctw = getenv("GRPP_CTW_MIN_PARTS");
         const long long ctw_min = ctw ? atoll(ctw) : CTW_MIN_PARTS;
         if (P->n_parts_small < ctw_min || P->n_parts_small == 0) {
-            std::stable_sort(run.begin(), run.end(), [&](const PartItem &x, const PartItem &y) { return cost(x) > cost(y); });
+            std::stable_sort(run.begin(), run.end(), [&](const PartItem &x, const PartItem &y) {
+                const long long cxv = cost(x), cyv = cost(y);
+                return cxv != cyv ? cxv > cyv : loc(x, y);
+            });
             P->n_parts_small = 0;
         }
         rc |= dput(P, &P->d_parts_run, run);
