@@ -177,6 +177,16 @@ __device__ __forceinline__ double int_gauss_s(int n, double a)
     return fk / (2.0 * pow(a, k + 1));
 }
 
+// Workgroup b of a launch runs on XCD b % 8 (the dispatcher deals workgroups round-robin over the
+// eight XCDs, each with its own L2).  This maps b to a work index so that, in every full group of
+// 8 * C workgroups, each XCD takes C consecutive work indices (neighbours in the launch order share
+// inputs); the indices of a last, partial group are kept.  A bijection of [0, n).
+__device__ __forceinline__ int xcd_chunked(int b, int n, int C)
+{
+    const int G = 8 * C, g0 = (b / G) * G;
+    return g0 + G <= n ? g0 + (b % 8) * C + (b / 8) % C : b;
+}
+
 constexpr int RT = 64;   // radial tile edge (entries)
 constexpr int RK = 32;   // grid points per K step
 constexpr int TS_MAX = 2048;   // upper bound of the LDS doubles for the (m, i, k2) intermediate

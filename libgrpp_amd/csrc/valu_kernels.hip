@@ -699,7 +699,9 @@ __global__ __launch_bounds__(CT_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8
 {
     // dynamic LDS sized by the plan: Q block (qs_len doubles) + (i, m, k2) intermediate (ts_len)
     extern __shared__ double ct_lds[];
-    contract_part<CT_THREADS, 256 / CT_THREADS, false>(partlist[blockIdx.x], items, J, T, partial, parts, nm_max, ts_len, ct_lds,
+    // runs of 32 parts per XCD: neighbours in the launch order read the same bra projectors
+    const int b = xcd_chunked(blockIdx.x, gridDim.x, 32);
+    contract_part<CT_THREADS, 256 / CT_THREADS, false>(partlist[b], items, J, T, partial, parts, nm_max, ts_len, ct_lds,
                                      ct_lds + qs_len, threadIdx.x);
 }
 
@@ -712,7 +714,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                                                   int ts_len)
 {
     extern __shared__ double ct_lds[];
-    const int w = threadIdx.x >> 6, p = blockIdx.x * 4 + w;
+    const int w = threadIdx.x >> 6, p = xcd_chunked(blockIdx.x, gridDim.x, 32) * 4 + w;
     if (p >= n_parts) return;   // whole waves leave; nothing below synchronises the block
     double *base = ct_lds + (size_t) w * (qs_len + ts_len);
     contract_part<64, 1, true>(partlist[p], items, J, T, partial, parts, nm_max, ts_len, base, base + qs_len,
