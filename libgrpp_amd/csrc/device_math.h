@@ -8,6 +8,7 @@ namespace grpp {
 
 struct DevTables {
     const double *r, *w, *rpow, *bes, *rfun, *boys;
+    const double *bes_t;      // bes in order-major layout [l][node][5] (grid tabulation)
     const double *besp;       // Taylor coefficients over k! (Horner form) of K_l and K_{l-1} per node and l,
                               // 80 bytes each: steering-only evaluations
     const double *rsh;

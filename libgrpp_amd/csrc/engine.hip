@@ -1137,6 +1137,8 @@ static int device_tables(int Lmax, int nmax, DevTables *out, bool no_alloc = fal
         D.t.rpow = p;
         if (upload(H.bes, &p, D.bufs)) return 1;
         D.t.bes = p;
+        if (upload(H.bes_t, &p, D.bufs)) return 1;
+        D.t.bes_t = p;
         if (upload(H.besp, &p, D.bufs)) return 1;
         D.t.besp = p;
         if (upload(H.rfun, &p, D.bufs)) return 1;

@@ -157,6 +157,11 @@ static void build(HostTables &T)
     for (int i = 0; i < BES_NPT; i++)
         for (int l = 0; l < BES_NL; l++)
             for (int k = 0; k < 5; k++) T.bes[((size_t) i * BES_NL + l) * 5 + k] = F(i, l, k);
+    // order-major copy: lanes at neighbouring grid points read neighbouring nodes of one order
+    T.bes_t.assign((size_t) BES_NPT * BES_NL * 5, 0.0);
+    for (int l = 0; l < BES_NL; l++)
+        for (int i = 0; i < BES_NPT; i++)
+            for (int k = 0; k < 5; k++) T.bes_t[((size_t) l * BES_NPT + i) * 5 + k] = F(i, l, k);
     // both orders of a Bessel ratio in one 80-byte entry (16-byte aligned): the steering
     // evaluations gather them with five 16-byte loads per lane
     T.besp.assign((size_t) BES_NPT * BES_NL * 10, 0.0);

@@ -28,6 +28,7 @@ struct HostTables {
     // (angular_integrals.c:35-56 without the harmonic of k), [(lam^2 + mu + lam) * NUC_NIJK + simplex(I,J,K)]
     std::vector<double> w1;
     std::vector<double> bes;                  // [BES_NPT][BES_NL][5]
+    std::vector<double> bes_t;                // the same, [BES_NL][BES_NPT][5] (order-major)
     std::vector<double> besp;                 // [BES_NPT][BES_NL][10]: Taylor coefficient k over k! (Horner form)
                                               // of K_l (0..4), then of K_{l-1} (K_1 for l = 0) (5..9)
     std::vector<double> rfun;                 // [BES_NL][BES_NL]

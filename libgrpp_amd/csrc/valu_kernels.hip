@@ -41,11 +41,13 @@ __global__ __launch_bounds__(256) void k_tab_F(const TabItem *__restrict__ items
         if (x > 1e-7 && x <= 16.0) {
             const int i0 = (int) (rint(x * 100) + 1e-5);
             const double ddx = x - i0 * 0.01, dx2 = ddx * ddx, dx3 = dx2 * ddx, dx4 = dx3 * ddx;
-            const double *row = T.bes + (size_t) i0 * BES_NL * 5;
+            // order-major table: the lanes of a wave, at neighbouring grid points, read neighbouring
+            // nodes of one order (fewer cache lines per load than node-major rows 1280 bytes apart)
+            const double *row = T.bes_t + (size_t) i0 * 5;
 #pragma unroll
             for (int l = 0; l <= MAXLAM; l++) {
                 if (l <= it.lam) {
-                    const double *d = row + l * 5;
+                    const double *d = row + (size_t) l * BES_NPT * 5;
                     const double K = d[0] + d[1] * ddx + d[2] * dx2 * 0.5 + d[3] * dx3 * (1.0 / 6.0) +
                                      d[4] * dx4 * (1.0 / 24.0);
                     acc[l] += e * K;
