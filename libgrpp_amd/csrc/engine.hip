@@ -2028,8 +2028,10 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
         launch_tab_F(P->n_tab, sd, P->d_tab, P->J, P->T, P->d_F);
     }
     HIPCHK(rec(P->ev[5], sd));
+    // one wave per projector item: most items have a few dozen outputs, so wider workgroups idled
+    // (every output is one thread's sum: the width changes no result)
     if (P->n_proj > 0)
-        hipLaunchKernelGGL(k_proj, dim3(P->n_proj), dim3(256), (size_t) P->proj_om * sizeof(double), sd, P->d_proj,
+        hipLaunchKernelGGL(k_proj, dim3(P->n_proj), dim3(64), (size_t) P->proj_om * sizeof(double), sd, P->d_proj,
                            P->J, P->T, P->d_P);
     if (P->n_delta_centers > 0)
         hipLaunchKernelGGL(k_delta, dim3(P->n_delta_centers), dim3(64), 0, sd, P->d_delta_centers,
