@@ -2036,8 +2036,8 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
     if (P->n_delta_centers > 0)
         hipLaunchKernelGGL(k_delta, dim3(P->n_delta_centers), dim3(64), 0, sd, P->d_delta_centers,
                            P->n_delta_centers, P->J, P->d_delta);
-    if (P->n_ocv > 0)
-        hipLaunchKernelGGL(k_ocv, dim3(P->n_ocv), dim3(256), (size_t) (P->ocv_scratch + 2 * P->ocv_nel) * sizeof(double),
+    if (P->n_ocv > 0)   // one wave per item as well (the MMD sums are per thread)
+        hipLaunchKernelGGL(k_ocv, dim3(P->n_ocv), dim3(64), (size_t) (P->ocv_scratch + 2 * P->ocv_nel) * sizeof(double),
                            sd, P->d_ocvi, P->J, P->T, P->d_OCV, P->ocv_scratch, P->ocv_nel);
     HIPCHK(rec(P->evs[1], sd));
     if (!front) {
