@@ -84,8 +84,11 @@ extern "C" const char *libgrpp_last_error(void) { return g_err; }
 // cancellation of the binomial expansion.
 __global__ __launch_bounds__(256) void k_canon(const TileItem *__restrict__ tiles, const QJob *__restrict__ jobs,
                                                const RowDesc *__restrict__ rows, DevJob J,
-                                               const double *__restrict__ Q, double *__restrict__ Qc)
+                                               const double *__restrict__ Q, double *__restrict__ Qc,
+                                               int upper_only)
 {
+    // upper_only: every work item has shell a <= shell b, so the contraction reads only the entries
+    // whose row shell is at most their column shell; the others are not written
     const TileItem t = tiles[blockIdx.x];
     const QJob jb = jobs[t.job];
     const RowDesc *R = rows + jb.rows;
@@ -101,6 +104,7 @@ __global__ __launch_bounds__(256) void k_canon(const TileItem *__restrict__ tile
         }
         if (row >= jb.M || col >= jb.M) continue;
         const RowDesc ra = R[row], rb = R[col];
+        if (upper_only && ra.shell > rb.shell) continue;
         const int la = J.sh[ra.shell].L, lb = J.sh[rb.shell].L, N = ra.n + rb.n;
         const bool a_first = (ra.shell < rb.shell) || (ra.shell == rb.shell && ra.lam <= rb.lam);
         const int cn = a_first ? dev_canon_nA(L, la, lb, ra.lam, rb.lam, N) : dev_canon_nA_max(L, la, lb, ra.lam, rb.lam, N);
@@ -1255,6 +1259,7 @@ struct eng_plan {
     double *d_partial_ct = nullptr;  // per-part semilocal, spin-orbit and outercore blocks (k_contract)
     int n_parts = 0;
     int n_parts_small = 0;           // parts of <= 64 Cartesian pairs: the tail of d_parts_run (k_contract_w)
+    int canon_upper = 0;             // every item has a <= b: k_canon writes the upper shell triangle only
     unsigned long long *d_counters = nullptr;
     eng_tuning_t tune{1, 128, 256, 0, 4};   // launch shapes and stream layout; none of them changes a result
     hipEvent_t ev[8] = {};   // execution phases on the caller's stream: see eng_plan_execute   // execution phases on the caller's stream: see eng_plan_execute
@@ -1795,6 +1800,9 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     rc |= dput(P, &P->d_ocvi, ocv);
     rc |= dput(P, &P->d_delta_centers, delta_centers);
     rc |= dput(P, &P->d_items, witems);
+    P->canon_upper = 1;
+    for (const WorkItem &w : witems)
+        if (w.a > w.b) P->canon_upper = 0;
     rc |= dput(P, &P->d_parts, partlist);
     {
         // launch order of type-1 and the contraction: the costliest (largest Cartesian blocks)
@@ -2072,7 +2080,7 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
     HIPCHK(rec(P->ev[3], st));
     if (P->n_tiles > 0)
         hipLaunchKernelGGL(k_canon, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles, P->d_jobs, P->d_rows, P->J, P->d_Q,
-                           P->d_Qc);
+                           P->d_Qc, P->canon_upper);
     HIPCHK(rec(P->ev[4], st));
     HIPCHK(wait(st, P->evs[1]));   // the contraction reads projectors and outercore vectors
     HIPCHK(rec(P->ev[6], st));
@@ -2743,8 +2751,9 @@ extern "C" int eng_plan_set_predictions(eng_plan_t *P, int n, const float *pred)
 }
 
 // diagnostics: copies a radial buffer of the last execute (0 = Q in the physical row order, 1 =
-// canonical Q and 2 = screening estimates in the logical order) and the row permutation (logical
-// -> physical) to the host
+// canonical Q -- only the entries the contraction reads: the upper shell triangle when every item
+// has a <= b -- and 2 = screening estimates in the logical order) and the row permutation
+// (logical -> physical) to the host
 extern "C" int eng_plan_debug_table(eng_plan_t *P, int which, double *out, int *rowpos, long long *qoff, int *M,
                                     int *rows)
 {
