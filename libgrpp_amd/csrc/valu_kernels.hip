@@ -629,6 +629,26 @@ __device__ __forceinline__ void contract_part(const PartItem pt, const WorkItem 
                     const double *SB = J.OCV + J.ocv_tab[(long long) (c.ob + oi) * J.nshell + it.b];
                     const double *UB = SB + ncB * nm;
                     const double *Lm = T.lmat + T.lmat_off[L];
+                    // The ket-side L-products sum_m2 L_eta[m1][m2] S_B[j][m2] (and of U_B) depend on j,
+                    // eta and m1 only: formed once per (j, eta, m1), with the same sums, in the LDS
+                    // the channels used (free by now), instead of once per output, when they fit
+                    double *W = Qs;
+                    const int avail = (int) (Ts - Qs) + ts_len, nlb = ncB * 3 * nm;
+                    const bool pre1 = oc1 && 2 * nlb <= avail;
+                    if (pre1) {
+                        for (int e = tid; e < nlb; e += CT_THREADS) {
+                            const int m1 = e % nm, je = e / nm, eta = je % 3, j = je / 3;
+                            const double *Le = Lm + eta * nm * nm;
+                            double lsb = 0.0, lub = 0.0;
+                            for (int m2 = 0; m2 < nm; m2++) {
+                                lsb += Le[m1 * nm + m2] * SB[j * nm + m2];
+                                lub += Le[m1 * nm + m2] * UB[j * nm + m2];
+                            }
+                            W[e] = lsb;
+                            W[nlb + e] = lub;
+                        }
+                        sync();
+                    }
 #pragma unroll
                     for (int q = 0; q < CT_EPT; q++) {
                         const int e = tid + q * CT_THREADS;
@@ -644,21 +664,46 @@ __device__ __forceinline__ void contract_part(const PartItem pt, const WorkItem 
                                 double so = 0.0;
                                 for (int m1 = 0; m1 < nm; m1++) {
                                     double lsb = 0.0, lub = 0.0;
-                                    for (int m2 = 0; m2 < nm; m2++) {
-                                        lsb += Le[m1 * nm + m2] * SB[j * nm + m2];
-                                        lub += Le[m1 * nm + m2] * UB[j * nm + m2];
+                                    if (pre1) {
+                                        lsb = W[(j * 3 + eta) * nm + m1];
+                                        lub = W[nlb + (j * 3 + eta) * nm + m1];
+                                    } else {
+                                        for (int m2 = 0; m2 < nm; m2++) {
+                                            lsb += Le[m1 * nm + m2] * SB[j * nm + m2];
+                                            lub += Le[m1 * nm + m2] * UB[j * nm + m2];
+                                        }
                                     }
                                     so += UA[i * nm + m1] * lsb + SA[i * nm + m1] * lub;
                                 }
                                 acc[1 + eta][q] += ef * so;
                             }
                         }
-                        // part 2: -P_nlj [(U + U')/2] P_n'lj for OC entries of equal (L, J)
-                        for (int oj = 0; oc2 && oj < no; oj++) {
-                            if (oc_pair && (oi != c.oc_i || oj != c.oc_j)) continue;
-                            const double d = J.delta[c.delta_off + oi * no + oj];
-                            if (d == 0.0) continue;
-                            const double *SB2 = J.OCV + J.ocv_tab[(long long) (c.ob + oj) * J.nshell + it.b];
+                    }
+                    // part 2: -P_nlj [(U + U')/2] P_n'lj for OC entries of equal (L, J).  Each output
+                    // still takes part 1 and then the oj terms in order: the loops over outputs and
+                    // oj are exchanged so that the L-products of an oj are formed once
+                    for (int oj = 0; oc2 && oj < no; oj++) {
+                        if (oc_pair && (oi != c.oc_i || oj != c.oc_j)) continue;
+                        const double d = J.delta[c.delta_off + oi * no + oj];
+                        if (d == 0.0) continue;
+                        const double *SB2 = J.OCV + J.ocv_tab[(long long) (c.ob + oj) * J.nshell + it.b];
+                        const bool pre2 = nlb <= avail;
+                        if (pre2) {
+                            sync();   // the previous products are read by every thread first
+                            for (int e = tid; e < nlb; e += CT_THREADS) {
+                                const int m1 = e % nm, je = e / nm, eta = je % 3, j = je / 3;
+                                const double *Le = Lm + eta * nm * nm;
+                                double l2 = 0.0;
+                                for (int m2 = 0; m2 < nm; m2++) l2 += Le[m1 * nm + m2] * SB2[j * nm + m2];
+                                W[e] = l2;
+                            }
+                            sync();
+                        }
+#pragma unroll
+                        for (int q = 0; q < CT_EPT; q++) {
+                            const int e = tid + q * CT_THREADS;
+                            if (e >= nel) break;
+                            const int i = r0 + e / ncB, j = e % ncB;
                             double s2 = 0.0;
                             for (int m = 0; m < nm; m++) s2 += SA[i * nm + m] * SB2[j * nm + m];
                             acc[0][q] += (-1.0) * d * af * s2;
@@ -668,13 +713,18 @@ __device__ __forceinline__ void contract_part(const PartItem pt, const WorkItem 
                                 double so = 0.0;
                                 for (int m1 = 0; m1 < nm; m1++) {
                                     double l2 = 0.0;
-                                    for (int m2 = 0; m2 < nm; m2++) l2 += Le[m1 * nm + m2] * SB2[j * nm + m2];
+                                    if (pre2) {
+                                        l2 = W[(j * 3 + eta) * nm + m1];
+                                    } else {
+                                        for (int m2 = 0; m2 < nm; m2++) l2 += Le[m1 * nm + m2] * SB2[j * nm + m2];
+                                    }
                                     so += SA[i * nm + m1] * l2;
                                 }
                                 acc[1 + eta][q] += (-1.0) * d * ef * so;
                             }
                         }
                     }
+                    sync();   // W is rewritten by the next OC entry
                 }
             }
         }
