@@ -635,6 +635,18 @@ __device__ __forceinline__ void contract_part(const PartItem pt, const WorkItem 
                     double *W = Qs;
                     const int avail = (int) (Ts - Qs) + ts_len, nlb = ncB * 3 * nm;
                     const bool pre1 = oc1 && 2 * nlb <= avail;
+                    // the OC vectors of A and B (S then U, contiguous) copied beside them when they fit
+                    const int nva = 2 * ncA * nm, nvb = 2 * ncB * nm, vbase = 2 * nlb;
+                    const bool stage = vbase + nva + nvb <= avail;
+                    if (stage) {
+                        for (int e = tid; e < nva; e += CT_THREADS) W[vbase + e] = SA[e];
+                        for (int e = tid; e < nvb; e += CT_THREADS) W[vbase + nva + e] = SB[e];
+                        SA = W + vbase;
+                        UA = SA + ncA * nm;
+                        SB = W + vbase + nva;
+                        UB = SB + ncB * nm;
+                        sync();
+                    }
                     if (pre1) {
                         for (int e = tid; e < nlb; e += CT_THREADS) {
                             const int m1 = e % nm, je = e / nm, eta = je % 3, j = je / 3;
@@ -690,6 +702,11 @@ __device__ __forceinline__ void contract_part(const PartItem pt, const WorkItem 
                         const bool pre2 = nlb <= avail;
                         if (pre2) {
                             sync();   // the previous products are read by every thread first
+                            if (stage) {   // S_B' over the copy of U_B (part 2 reads S_A and S_B' only)
+                                for (int e = tid; e < ncB * nm; e += CT_THREADS) W[vbase + nva + ncB * nm + e] = SB2[e];
+                                SB2 = W + vbase + nva + ncB * nm;
+                                sync();
+                            }
                             for (int e = tid; e < nlb; e += CT_THREADS) {
                                 const int m1 = e % nm, je = e / nm, eta = je % 3, j = je / 3;
                                 const double *Le = Lm + eta * nm * nm;
