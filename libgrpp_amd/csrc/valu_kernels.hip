@@ -509,8 +509,42 @@ __device__ __forceinline__ void contract_part(const PartItem pt, const WorkItem 
             const double CA2 = CAx * CAx + CAy * CAy + CAz * CAz, CB2 = CBx * CBx + CBy * CBy + CBz * CBz;
             const bool onec = CA2 < 1e-14 && CB2 < 1e-14;
             // ---- semilocal and spin-orbit channels ----
-            for (int pi = c.pb; pi < c.pe; pi++) {
-                const DPot p = J.pot[pi];
+            // one-wave parts: each channel's record and block offsets are loaded by one lane (the
+            // dependent loads of up to 64 channels in flight at once) and read back with readlane
+            for (int pb0 = c.pb; pb0 < c.pe; pb0 += 64) {
+            const int nch = min(64, c.pe - pb0), lane = tid & 63;
+            int m_L = 0, m_kind = 0, m_M = 0, m_ra = 0, m_rb = 0;
+            long long m_qoff = 0, m_pa = 0, m_pb = 0, m_plb = 0;
+            double m_scale = 0.0;
+            if (WAVE && lane < nch) {
+                const DPot pl = J.pot[pb0 + lane];
+                const long long so = (long long) pl.slot * J.nshell;
+                m_L = pl.L;
+                m_kind = pl.kind;
+                m_M = pl.M;
+                m_qoff = pl.qoff;
+                m_scale = pl.scale;
+                m_ra = J.slot_rowoff[so + it.a];
+                m_rb = J.slot_rowoff[so + it.b];
+                m_pa = (pl.kind == K_SO ? J.slot_pb3 : J.slot_pb2)[so + it.a];
+                m_pb = J.slot_poff[so + it.b];
+                m_plb = pl.kind == K_SO ? J.slot_ploff[so + it.b] : 0;
+            }
+            auto rl = [](int v, int k) { return __builtin_amdgcn_readlane(v, k); };
+            auto rl64 = [](long long v, int k) {
+                const int lo = __builtin_amdgcn_readlane((int) (v & 0xffffffffLL), k);
+                const int hi = __builtin_amdgcn_readlane((int) (v >> 32), k);
+                return ((long long) hi << 32) | (unsigned int) lo;
+            };
+            for (int pi = pb0; pi < pb0 + nch; pi++) {
+                const int ch = pi - pb0;
+                DPot p;
+                if (WAVE) {
+                    p.L = rl(m_L, ch);
+                    p.kind = rl(m_kind, ch);
+                } else {
+                    p = J.pot[pi];
+                }
                 const bool is_so = (p.kind == K_SO);
                 if (is_so && !(parts & 4)) continue;
                 if (!is_so && !(parts & 2)) continue;
@@ -520,6 +554,7 @@ __device__ __forceinline__ void contract_part(const PartItem pt, const WorkItem 
                 bool nz = false;
                 if (onec) {
                     // analytic one-centre branch (radial_type2_integral.c:126-155): only lambda1 = lambda2 = 0
+                    if (WAVE) p = J.pot[pi];
                     for (int e = tid; e < KA * KB; e += CT_THREADS) Qs[e] = 0.0;
                     sync();
                     if (tid < (la + 1) * (lb + 1)) {
@@ -543,21 +578,32 @@ __device__ __forceinline__ void contract_part(const PartItem pt, const WorkItem 
                         }
                     }
                 } else {
-                    const int ra = J.slot_rowoff[(long long) p.slot * J.nshell + it.a];
-                    const int rb = J.slot_rowoff[(long long) p.slot * J.nshell + it.b];
+                    const int ra = WAVE ? rl(m_ra, ch) : J.slot_rowoff[(long long) p.slot * J.nshell + it.a];
+                    const int rb = WAVE ? rl(m_rb, ch) : J.slot_rowoff[(long long) p.slot * J.nshell + it.b];
+                    const int M = WAVE ? rl(m_M, ch) : p.M;
+                    const long long qoff = WAVE ? rl64(m_qoff, ch) : p.qoff;
                     for (int e = tid; e < KA * KB; e += CT_THREADS) {
                         const int k1 = e / KB, k2 = e % KB;
-                        const double v = J.Q[p.qoff + (long long) (ra + k1) * p.M + rb + k2];   // logical order
+                        const double v = J.Q[qoff + (long long) (ra + k1) * M + rb + k2];   // logical order
                         Qs[e] = v;
                         nz = nz || v != 0.0;
                     }
                 }
                 // a block of zeros adds exactly zero to every output (acc is never -0): skip the channel
                 if (!sync_or(nz)) continue;
-                const double *PA = J.P + (is_so ? J.slot_pb3 : J.slot_pb2)[(long long) p.slot * J.nshell + it.a];
-                const double *PB = J.P + J.slot_poff[(long long) p.slot * J.nshell + it.b];
-                const double *PLB = is_so ? J.P + J.slot_ploff[(long long) p.slot * J.nshell + it.b] : nullptr;
-                const double fac = FOURPI2 * p.scale;
+                const double *PA, *PB, *PLB;
+                double fac;
+                if (WAVE) {
+                    PA = J.P + rl64(m_pa, ch);
+                    PB = J.P + rl64(m_pb, ch);
+                    PLB = is_so ? J.P + rl64(m_plb, ch) : nullptr;
+                    fac = FOURPI2 * __longlong_as_double(rl64(__double_as_longlong(m_scale), ch));
+                } else {
+                    PA = J.P + (is_so ? J.slot_pb3 : J.slot_pb2)[(long long) p.slot * J.nshell + it.a];
+                    PB = J.P + J.slot_poff[(long long) p.slot * J.nshell + it.b];
+                    PLB = is_so ? J.P + J.slot_ploff[(long long) p.slot * J.nshell + it.b] : nullptr;
+                    fac = FOURPI2 * p.scale;
+                }
                 // T[i][m][k2] = sum_k1 P^A[i][m][k1] Q[k1][k2] for all m at once; row i of T is then
                 // contiguous over (m, k2), as row j of the ket projectors is, so each output is one
                 // dot product of length nm * KB (the same terms in the same order as a loop over m
@@ -610,6 +656,7 @@ __device__ __forceinline__ void contract_part(const PartItem pt, const WorkItem 
                     }
                 }
                 sync();
+            }
             }
             // ---- outercore projectors: part 1 (outercore_integrals.c:147-270) and part 2
             // (:277-390); a part-2-only job may restrict part 2 to one (oi, oj) term ----
