@@ -1270,6 +1270,7 @@ struct eng_plan {
     hipEvent_t gev[2] = {};       // around a graph replay
     hipEvent_t evs[3] = {};       // phases on the side stream
     hipEvent_t evt1 = nullptr;    // start of type-1
+    hipEvent_t evr = nullptr;     // start of the radial product (after its wait for the grid tables)
     // statistics
     long long n_tasks = 0, n_integrals = 0, n_entries = 0;
     double flops_fixed = 0.0;    // contraction + type-1 + projector work (grid independent)
@@ -1974,6 +1975,7 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
         for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&P->ev[i]));
         for (int i = 0; i < 3; i++) HIPCHK(hipEventCreate(&P->evs[i]));
         HIPCHK(hipEventCreate(&P->evt1));
+        HIPCHK(hipEventCreate(&P->evr));
         HIPCHK(hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking));
     }
     if (flags & ENG_F_GRAPH) {
@@ -2007,8 +2009,8 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
     // projectors (s1), the final reduction for the type-1 slices (s2).  Layout 1 (an autotuner
     // choice) runs the tables, projectors and outercore vectors on the caller's stream ahead of the
     // screening instead, and type-1 alone on the side stream from the start.
-    // Events: caller 0 start = 1 | screening | 2 | radial | 3 | canonical | 4 ... 6 | contraction,
-    // reduction | 7; s0 | tables | 5 | projectors, outercore | s1 on the tables' stream;
+    // Events: caller 0 start = 1 | screening | 2 (wait for the tables) r | radial | 3 | canonical | 4 ...
+    // 6 | contraction, reduction | 7; s0 | tables | 5 | projectors, outercore | s1 on the tables' stream;
     // side t1 | type-1 | s2.
     // serial plans (ENG_F_ARENA) run every kernel in this order on the caller's stream, which
     // orders them as the events below would; they own no stream or event
@@ -2061,8 +2063,9 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
         const int sp = P->tune.screen_probe, ss = P->tune.screen_split;
         launch_screen(sp, ss, P->n_tiles, st, P->d_tiles, P->d_jobs, P->d_rows, P->J, P->T, P->d_scr, P->d_counters);
     }
+    HIPCHK(rec(P->ev[2], st));    // end of the screening
     HIPCHK(wait(st, P->ev[5]));   // grid tables of the side stream
-    HIPCHK(rec(P->ev[2], st));
+    HIPCHK(rec(P->evr, st));
     if (P->n_tiles > 0) {
         if (P->radial_mfma && P->d_lev)
             hipLaunchKernelGGL(k_radial_mfma<true>, dim3(P->n_tiles), dim3(256), 0, st, P->d_tiles_radial, P->d_jobs,
@@ -2171,7 +2174,7 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
 
 static int eager_phase_times(eng_plan_t *P)
 {
-    const hipEvent_t *pb[7] = {&P->evs[0], &P->ev[1], &P->ev[2], &P->ev[3], &P->ev[5], &P->evt1, &P->ev[6]};
+    const hipEvent_t *pb[7] = {&P->evs[0], &P->ev[1], &P->evr, &P->ev[3], &P->ev[5], &P->evt1, &P->ev[6]};
     const hipEvent_t *pe[7] = {&P->ev[5], &P->ev[2], &P->ev[3], &P->ev[4], &P->evs[1], &P->evs[2], &P->ev[7]};
     for (int i = 0; i < 7; i++) {
         float ms = 0;
@@ -2608,6 +2611,7 @@ extern "C" void eng_plan_destroy(eng_plan_t *P)
     for (int i = 0; i < 3; i++)
         if (P->evs[i]) (void) hipEventDestroy(P->evs[i]);
     if (P->evt1) (void) hipEventDestroy(P->evt1);
+    if (P->evr) (void) hipEventDestroy(P->evr);
     if (P->side) (void) hipStreamDestroy(P->side);
     if (P->gexec) (void) hipGraphExecDestroy(P->gexec);
     if (P->graph) (void) hipGraphDestroy(P->graph);
