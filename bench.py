@@ -67,19 +67,9 @@ def fp64_counters(workload: str, step_s: float):
     + ADD + TRANS) + 512 x MFMA_MOPS_F64, an upper bound (every lane counted active).  Returns the
     per-kernel figures and the whole build's fraction of the FP64 peak at the measured step time;
     None when no summary of that workload is present."""
-    path = os.path.join(ROOT, "profiles", f"counters_{workload}.txt")
-    if not os.path.exists(path):
+    rows, path = _counter_rows(workload)
+    if not rows:
         return None
-    rows, cols = {}, None
-    for line in open(path):
-        f = line.split()
-        if len(f) >= 2 and f[0] == "kernel":
-            cols = f[1:]
-        elif cols and len(f) == len(cols) + 1 and f[0].startswith("k_"):
-            try:
-                rows.setdefault(f[0], {}).update({c: float(v) for c, v in zip(cols, f[1:])})
-            except ValueError:
-                pass
     per = {}
     for k, r in rows.items():
         if "VALU_FMA_F64" not in r:
@@ -94,6 +84,38 @@ def fp64_counters(workload: str, step_s: float):
             "per_kernel": per, "build_achieved_tflops": total / step_s / 1e12,
             "build_frac": total / step_s / 1e12 / FP64_PEAK_TFLOPS,
             "note": "upper bound: 64 active lanes per wave instruction"}
+
+
+def _counter_rows(workload: str):
+    """per-kernel rows of the committed rocprofv3 counter summary of ``workload`` (both tables)"""
+    path = os.path.join(ROOT, "profiles", f"counters_{workload}.txt")
+    if not os.path.exists(path):
+        return None, path
+    rows, cols = {}, None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 2 and f[0] == "kernel":
+            cols = f[1:]
+        elif cols and len(f) == len(cols) + 1 and f[0].startswith("k_"):
+            try:
+                rows.setdefault(f[0], {}).update({c: float(v) for c, v in zip(cols, f[1:])})
+            except ValueError:
+                pass
+    return rows, path
+
+
+def valu_issue(workload: str, kernel: str, seconds: float):
+    """VALU issue load of ``kernel``: its wave instructions per launch (SQ_INSTS_VALU from the
+    committed counter summary) at 4 cycles each (a wave64 instruction on a 16-lane SIMD; FP64 FMA,
+    MUL and ADD issue at that rate, transcendentals slower, some 32-bit operations faster) over the
+    issue capacity of the measured duration: 1024 SIMDs at the 2.4 GHz peak engine clock.  None when
+    no summary of that workload is present."""
+    rows, path = _counter_rows(workload)
+    if not rows or kernel not in rows or "VALU" not in rows[kernel] or seconds <= 0:
+        return None
+    v = rows[kernel]["VALU"]
+    return {"source": os.path.relpath(path, ROOT), "valu_wave_insts": v, "cycles_per_inst": 4,
+            "frac": v * 4.0 / (seconds * 1024 * 2.4e9)}
 
 
 def host_info():
@@ -449,6 +471,7 @@ def main():
     final = out if learned_info else plain_out
     line["parity"] = parity_fields(args.workload, sysm, final)
     line["fp64_counters"] = fp64_counters(args.workload, ms / 1e3)
+    line["roofline"]["valu_issue"] = valu_issue(args.workload, "k_screen", scr)
     line["autotune"] = tuning_plain
     if graph_info:
         line["graph_leg"] = graph_info
