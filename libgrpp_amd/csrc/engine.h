@@ -59,7 +59,10 @@ typedef struct {
 enum { ENG_F_STATS = 2, ENG_F_LEVELS = 4, ENG_F_GRAPH = 8,
        ENG_F_ARENA = 16 /* every device buffer carved from the device's static scratch array, no
                            stream or event of the plan's own: the build runs in order on the
-                           caller's stream (the allocation-free entry point) */ };
+                           caller's stream (the allocation-free entry point) */,
+       ENG_F_SERIAL = 32 /* no stream or event of the plan's own (buffers from the pool): the build
+                            runs in order on the caller's stream (one-task plans of the per-call
+                            entry points) */ };
 #define ENG_SCRATCH_BYTES (64ull << 20)   /* size of that static scratch array per device */
 #define ENG_NFEAT 12   /* level-predictor features per radial row */
 

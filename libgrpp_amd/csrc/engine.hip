@@ -1360,11 +1360,20 @@ static int dalloc(eng_plan *P, T **p, size_t n)
     return 0;
 }
 
+// the stream a plan's uploads are ordered on while it is created (per host thread; null: hipMemcpy).
+// The per-call entry points set it to the calling thread's own stream, so that their uploads do not
+// wait for other threads' work on the device
+static thread_local hipStream_t t_upload_stream = nullptr;
+
 template <class T>
 static int dput(eng_plan *P, T **p, const std::vector<T> &v)
 {
     if (dalloc(P, p, v.size())) return 1;
-    if (!v.empty()) HIPCHK(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (v.empty()) return 0;
+    if (t_upload_stream)
+        HIPCHK(hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, t_upload_stream));
+    else
+        HIPCHK(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -1391,6 +1400,7 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
         }
         P->serial = true;
     }
+    if (flags & ENG_F_SERIAL) P->serial = true;
     P->nshell = nshell;
     P->ncenter = ncenter;
     P->parts = parts;
@@ -1987,6 +1997,8 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
         const char *k = getenv("GRPP_RADIAL_KERNEL");
         P->radial_mfma = !(k && strcmp(k, "valu") == 0);
     }
+    // uploads ordered on the caller's stream: done before the host vectors they read go away
+    if (t_upload_stream) HIPCHK(hipStreamSynchronize(t_upload_stream));
     *plan_out = P;
     return 0;
 }
@@ -2776,12 +2788,25 @@ extern "C" int eng_run(int nshell, libgrpp_shell_t *const *shells, int ncenter, 
                        const int *center_index, int nitem, const eng_item_t *items, long long out_len,
                        long long op_stride, int parts, double *out)
 {
+    // a one-task plan on the calling thread's own stream, in order (no stream or events of its own):
+    // calls from several host threads overlap on the device
     eng_plan_t *P = nullptr;
+    const hipStream_t st = hipStreamPerThread;
+    t_upload_stream = st;
     int rc = eng_plan_create(nshell, shells, ncenter, centers, nidx, center_index, nitem, items, out_len, op_stride,
-                             parts, 0, &P);
+                             parts, ENG_F_SERIAL, &P);
+    t_upload_stream = nullptr;
     if (rc) return rc;
-    rc = eng_plan_execute(P, nullptr);
-    if (!rc) rc = eng_plan_download(P, out);
+    rc = eng_plan_execute(P, st);   // synchronises st
+    if (!rc) {
+        const hipError_t e = hipMemcpyAsync(out, P->d_out, (size_t) 4 * P->op_stride * sizeof(double),
+                                            hipMemcpyDeviceToHost, st);
+        const hipError_t e2 = e == hipSuccess ? hipStreamSynchronize(st) : e;
+        if (e2 != hipSuccess) {
+            eng_set_error("HIP error %s in eng_run's download", hipGetErrorString(e2));
+            rc = 1;
+        }
+    }
     eng_plan_destroy(P);
     return rc;
 }
