@@ -40,8 +40,11 @@ def build(verbose: bool = False) -> str:
         ([HIPCC] + HIP_FLAGS + inc + ["-c", os.path.join(CSRC, "engine.hip"), "-o", os.path.join(BUILD, "engine.o")],
          os.path.join(BUILD, "engine.o")),
         # the screening, grid-tabulation and contraction kernels: without machine loop-invariant code
-        # motion, which hoisted their FP64 constants out of the loops and spilled them
-        ([HIPCC] + HIP_FLAGS + ["-mllvm", "-disable-machine-licm"] + inc +
+        # motion, which hoisted their FP64 constants out of the loops and spilled them, and with the
+        # iterative ILP scheduler (the same instructions, reordered: screening and contraction about
+        # 1.5 and 1 ms shorter on the ragged workload)
+        ([HIPCC] + HIP_FLAGS + ["-mllvm", "-disable-machine-licm", "-mllvm", "-amdgpu-sched-strategy=iterative-ilp"] +
+         inc +
          ["-c", os.path.join(CSRC, "valu_kernels.hip"), "-o", os.path.join(BUILD, "valu_kernels.o")],
          os.path.join(BUILD, "valu_kernels.o")),
         ([HIPCC] + HIP_FLAGS + inc + ["-c", os.path.join(CSRC, "mlp.hip"), "-o", os.path.join(BUILD, "mlp.o")],
