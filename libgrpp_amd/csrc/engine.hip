@@ -1229,6 +1229,8 @@ struct eng_plan {
     std::vector<std::pair<void *, size_t>> bufs;   // device buffers and their pooled sizes
     unsigned char *arena = nullptr;               // ENG_F_ARENA: buffers carved from the scratch array
     size_t arena_used = 0;
+    unsigned char *up_dev = nullptr, *up_host = nullptr;   // per-call plans: uploads staged in one block
+    size_t up_used = 0;
     bool serial = false;                          // one stream (the caller's), no events
     double *d_out = nullptr, *d_F = nullptr, *d_wU = nullptr, *d_Q = nullptr, *d_P = nullptr, *d_OCV = nullptr,
            *d_delta = nullptr, *d_scr = nullptr, *d_Qc = nullptr;
@@ -1365,9 +1367,48 @@ static int dalloc(eng_plan *P, T **p, size_t n)
 // wait for other threads' work on the device
 static thread_local hipStream_t t_upload_stream = nullptr;
 
+// Per-call plans stage their uploads: each buffer is carved from one pooled device block of
+// UPLOAD_BLOCK bytes and its data written at the same offset of a pinned host buffer of that size,
+// and the whole block goes up in one copy at the end of plan creation (about 34 small pageable
+// copies otherwise).  Pinned staging buffers are taken from a free list and returned after the
+// call; each stays allocated until the process ends (one per host thread that calls at once).
+constexpr size_t UPLOAD_BLOCK = (size_t) 4 << 20;
+static std::vector<unsigned char *> g_stage_free;   // under g_pool_mutex
+
+static unsigned char *stage_take()
+{
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mutex);
+        if (!g_stage_free.empty()) {
+            unsigned char *p = g_stage_free.back();
+            g_stage_free.pop_back();
+            return p;
+        }
+    }
+    void *p = nullptr;
+    return hipHostMalloc(&p, UPLOAD_BLOCK, hipHostMallocDefault) == hipSuccess ? (unsigned char *) p : nullptr;
+}
+
+static void stage_give(unsigned char *p)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_pool_mutex);
+    g_stage_free.push_back(p);
+}
+
+// the pinned staging buffer of the calling thread's per-call plan while it is created (null: none)
+static thread_local unsigned char *t_stage = nullptr;
+
 template <class T>
 static int dput(eng_plan *P, T **p, const std::vector<T> &v)
 {
+    const size_t bytes = (std::max<size_t>(1, v.size()) * sizeof(T) + 255) & ~(size_t) 255;
+    if (P->up_host && P->up_used + bytes <= UPLOAD_BLOCK) {
+        *p = (T *) (P->up_dev + P->up_used);
+        if (!v.empty()) memcpy(P->up_host + P->up_used, v.data(), v.size() * sizeof(T));
+        P->up_used += bytes;
+        return 0;
+    }
     if (dalloc(P, p, v.size())) return 1;
     if (v.empty()) return 0;
     if (t_upload_stream)
@@ -1401,6 +1442,16 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
         P->serial = true;
     }
     if (flags & ENG_F_SERIAL) P->serial = true;
+    if (t_upload_stream && t_stage && !P->arena) {
+        void *q = nullptr;
+        if (pool_alloc(P->device, UPLOAD_BLOCK, &q)) {
+            delete P;
+            return 1;
+        }
+        P->bufs.push_back({q, UPLOAD_BLOCK});
+        P->up_dev = (unsigned char *) q;
+        P->up_host = t_stage;
+    }
     P->nshell = nshell;
     P->ncenter = ncenter;
     P->parts = parts;
@@ -1997,7 +2048,10 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
         const char *k = getenv("GRPP_RADIAL_KERNEL");
         P->radial_mfma = !(k && strcmp(k, "valu") == 0);
     }
-    // uploads ordered on the caller's stream: done before the host vectors they read go away
+    // uploads ordered on the caller's stream: done before the host vectors (or the staging buffer)
+    // they read go away
+    if (P->up_used) HIPCHK(hipMemcpyAsync(P->up_dev, P->up_host, P->up_used, hipMemcpyHostToDevice, t_upload_stream));
+    P->up_host = nullptr;
     if (t_upload_stream) HIPCHK(hipStreamSynchronize(t_upload_stream));
     *plan_out = P;
     return 0;
@@ -2792,21 +2846,33 @@ extern "C" int eng_run(int nshell, libgrpp_shell_t *const *shells, int ncenter, 
     // calls from several host threads overlap on the device
     eng_plan_t *P = nullptr;
     const hipStream_t st = hipStreamPerThread;
+    unsigned char *stage = stage_take();   // null (no pinned memory): the plain uploads
     t_upload_stream = st;
+    t_stage = stage;
     int rc = eng_plan_create(nshell, shells, ncenter, centers, nidx, center_index, nitem, items, out_len, op_stride,
                              parts, ENG_F_SERIAL, &P);
     t_upload_stream = nullptr;
-    if (rc) return rc;
+    t_stage = nullptr;
+    if (rc) {
+        stage_give(stage);
+        return rc;
+    }
     rc = eng_plan_execute(P, st);   // synchronises st
     if (!rc) {
-        const hipError_t e = hipMemcpyAsync(out, P->d_out, (size_t) 4 * P->op_stride * sizeof(double),
+        // the download through the pinned buffer when it fits
+        const size_t bytes = (size_t) 4 * P->op_stride * sizeof(double);
+        const bool staged = stage && bytes <= UPLOAD_BLOCK;
+        const hipError_t e = hipMemcpyAsync(staged ? (void *) stage : (void *) out, P->d_out, bytes,
                                             hipMemcpyDeviceToHost, st);
         const hipError_t e2 = e == hipSuccess ? hipStreamSynchronize(st) : e;
         if (e2 != hipSuccess) {
             eng_set_error("HIP error %s in eng_run's download", hipGetErrorString(e2));
             rc = 1;
+        } else if (staged) {
+            memcpy(out, stage, bytes);
         }
     }
     eng_plan_destroy(P);
+    stage_give(stage);
     return rc;
 }
