@@ -35,6 +35,7 @@
 // are needed on the outputs; blocks of (B,A) are produced from (A,B) by (anti)symmetry.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1209,11 +1210,14 @@ extern "C" int eng_setup_device(void)
 
 extern "C" int eng_device_ok(void)
 {
+    static std::atomic<int> seen{0};   // a device, once found, stays for the process
+    if (seen.load(std::memory_order_relaxed)) return 1;
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
         eng_set_error("no HIP device available: the GRPP integrals run only on the GPU");
         return 0;
     }
+    seen.store(1, std::memory_order_relaxed);
     return 1;
 }
 
@@ -1233,6 +1237,7 @@ struct eng_plan {
     unsigned char *up_dev = nullptr, *up_host = nullptr;   // per-call plans: uploads staged in one block
     size_t up_used = 0;
     bool percall = false;                         // built by eng_run: one call, then destroyed
+    bool up_pageable = false;                     // a per-call upload went straight from a host vector
     bool serial = false;                          // one stream (the caller's), no events
     double *d_out = nullptr, *d_F = nullptr, *d_wU = nullptr, *d_Q = nullptr, *d_P = nullptr, *d_OCV = nullptr,
            *d_delta = nullptr, *d_scr = nullptr, *d_Qc = nullptr;
@@ -1413,8 +1418,10 @@ static int dput(eng_plan *P, T **p, const std::vector<T> &v)
     }
     if (dalloc(P, p, v.size())) return 1;
     if (v.empty()) return 0;
-    if (t_upload_stream)
+    if (t_upload_stream) {
+        P->up_pageable = true;
         HIPCHK(hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, t_upload_stream));
+    }
     else
         HIPCHK(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
     return 0;
@@ -2051,11 +2058,11 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
         const char *k = getenv("GRPP_RADIAL_KERNEL");
         P->radial_mfma = !(k && strcmp(k, "valu") == 0);
     }
-    // uploads ordered on the caller's stream: done before the host vectors (or the staging buffer)
-    // they read go away
+    // uploads ordered on the caller's stream: those from host vectors are done before the vectors go
+    // away; the staging buffer outlives the call (eng_run returns it after its final synchronisation)
     if (P->up_used) HIPCHK(hipMemcpyAsync(P->up_dev, P->up_host, P->up_used, hipMemcpyHostToDevice, t_upload_stream));
     P->up_host = nullptr;
-    if (t_upload_stream) HIPCHK(hipStreamSynchronize(t_upload_stream));
+    if (t_upload_stream && P->up_pageable) HIPCHK(hipStreamSynchronize(t_upload_stream));
     *plan_out = P;
     return 0;
 }
@@ -2206,7 +2213,8 @@ extern "C" int eng_plan_execute(eng_plan_t *P, void *stream_)
     } else if (P->serial) {
         const int rc = enqueue_build(P, st);
         if (rc) return rc;
-        HIPCHK(hipStreamSynchronize(st));
+        // a per-call plan's download follows on the same stream, and eng_run synchronises after it
+        if (!P->percall) HIPCHK(hipStreamSynchronize(st));
         for (int i = 0; i < 7; i++) P->phase_seconds[i] = 0.0;   // no events: not timed
         P->last_seconds = 0.0;
         P->last_radial_seconds = 0.0;
@@ -2863,7 +2871,7 @@ extern "C" int eng_run(int nshell, libgrpp_shell_t *const *shells, int ncenter, 
         stage_give(stage);
         return rc;
     }
-    rc = eng_plan_execute(P, st);   // synchronises st
+    rc = eng_plan_execute(P, st);   // enqueues only: the download below is ordered after it on st
     if (!rc) {
         // the download through the pinned buffer when it fits
         const size_t bytes = (size_t) 4 * P->op_stride * sizeof(double);
@@ -2877,6 +2885,8 @@ extern "C" int eng_run(int nshell, libgrpp_shell_t *const *shells, int ncenter, 
         } else if (staged) {
             memcpy(out, stage, bytes);
         }
+    } else {
+        (void) hipStreamSynchronize(st);   // what was enqueued ends before its buffers are pooled again
     }
     eng_plan_destroy(P);
     stage_give(stage);
