@@ -961,22 +961,67 @@ __device__ double onecenter_contracted(const DShell &b, const DShell &k, const D
     return sum;
 }
 
-__global__ void k_delta(const int *__restrict__ centers, int n, DevJob J, double *__restrict__ delta)
+// term q = (i * k.nprim + j) * nterm + t of onecenter_contracted's sum, as the factor x and the
+// integral u of its last product (the term is x * u), evaluated as the loop there evaluates it
+__device__ __forceinline__ void onecenter_term(const DShell &b, const DShell &k, const DPot &p, const DevJob &J, int q,
+                                               double &x, double &u)
 {
-    const int ci = blockIdx.x;
-    const DCenter c = J.ctr[centers[ci]];
-    const int no = c.oe - c.ob;
-    for (int e = threadIdx.x; e < no * no; e += blockDim.x) {
-        const int i = e / no, j = e % no;
-        const OcEntry oi = J.oce[c.ob + i], oj = J.oce[c.ob + j];
-        const DPot p1 = J.pot[oi.pot], p2 = J.pot[oj.pot];
-        double d = 0.0;
-        if (p1.L == p2.L && p1.J == p2.J) {
-            const DShell s1 = J.sh[oi.shell], s2 = J.sh[oj.shell];
-            d = 0.5 * (onecenter_contracted(s1, s2, p1, J) + onecenter_contracted(s1, s2, p2, J));
-        }
-        delta[c.delta_off + e] = d;
+    const int nt = p.nterm, i = q / (k.nprim * nt), r = q - i * (k.nprim * nt), j = r / nt, t = r - j * nt;
+    const int L = p.L;
+    const double ai = J.alpha[b.poff + i], Ni = rgto_norm(L, ai);
+    const double aj = J.alpha[k.poff + j], Nj = rgto_norm(L, aj);
+    const double f = Ni * Nj * J.craw[b.poff + i] * J.craw[k.poff + j];
+    const int n = J.tpow[p.toff + t];
+    const double a = ai + aj + J.tzeta[p.toff + t];
+    if (n % 2 == 0) {
+        const int kk = L + n / 2;
+        double df = 1.0;
+        for (int z = 2 * kk - 1; z > 1; z -= 2) df *= z;
+        u = df / (ldexp(1.0, kk + 1) * pow(a, kk)) * sqrt(M_PI / a);
+    } else {
+        const int kk = L + (n - 1) / 2;
+        double fk = 1.0;
+        for (int z = 2; z <= kk; z++) fk *= z;
+        u = fk / (2.0 * pow(a, kk + 1));
     }
+    x = f * J.tcoef[p.toff + t];
+}
+
+// One wave per (centre, outercore entry pair) (grid: centres x the largest entry count squared):
+// the lanes evaluate 64 terms of a sum at a time and lane 0 adds them in the sum's own order, so each
+// constant is bitwise onecenter_contracted's one-thread loop
+__global__ __launch_bounds__(64) void k_delta(const int *__restrict__ centers, int n, DevJob J,
+                                              double *__restrict__ delta)
+{
+    __shared__ double xs[64], us[64];
+    const DCenter c = J.ctr[centers[blockIdx.x]];
+    const int no = c.oe - c.ob, e = blockIdx.y, lane = threadIdx.x;
+    if (e >= no * no) return;   // the whole workgroup
+    const int i = e / no, j = e % no;
+    const OcEntry oi = J.oce[c.ob + i], oj = J.oce[c.ob + j];
+    const DPot p1 = J.pot[oi.pot], p2 = J.pot[oj.pot];
+    double d = 0.0;
+    if (p1.L == p2.L && p1.J == p2.J) {
+        const DShell s1 = J.sh[oi.shell], s2 = J.sh[oj.shell];
+        double sum[2];
+        for (int w = 0; w < 2; w++) {
+            const DPot p = w ? p2 : p1;
+            const int nq = s1.nprim * s2.nprim * p.nterm;
+            double s = 0.0;
+            for (int q0 = 0; q0 < nq; q0 += 64) {
+                if (q0 + lane < nq) onecenter_term(s1, s2, p, J, q0 + lane, xs[lane], us[lane]);
+                __syncthreads();
+                if (lane == 0) {
+                    const int m = nq - q0 < 64 ? nq - q0 : 64;
+                    for (int k = 0; k < m; k++) s += xs[k] * us[k];
+                }
+                __syncthreads();
+            }
+            sum[w] = s;
+        }
+        d = 0.5 * (sum[0] + sum[1]);
+    }
+    if (lane == 0) delta[c.delta_off + e] = d;
     (void) n;
 }
 
@@ -1259,6 +1304,7 @@ struct eng_plan {
     int n_ocv = 0;
     int *d_delta_centers = nullptr;
     int n_delta_centers = 0;
+    int delta_max_pairs = 0;   // the largest outercore entry count squared of a centre
     WorkItem *d_items = nullptr;
     int n_items = 0;
     PartItem *d_parts = nullptr;     // (item, centre) workgroups of the contraction, item order
@@ -1953,6 +1999,12 @@ extern "C" int eng_plan_create(int nshell, libgrpp_shell_t *const *shells, int n
     P->proj_om = proj_om;
     P->n_ocv = (int) ocv.size();
     P->n_delta_centers = (int) delta_centers.size();
+    for (int c : delta_centers) P->delta_max_pairs = std::max(P->delta_max_pairs, (ctr[c].oe - ctr[c].ob) * (ctr[c].oe - ctr[c].ob));
+    if (P->delta_max_pairs > 65535) {   // the launch's second grid dimension
+        eng_set_error("a centre has more than 255 outercore entries");
+        eng_plan_destroy(P);
+        return 1;
+    }
     P->n_items = nitem;
     P->n_parts = (int) partlist.size();
     P->nm_max = 2 * Lmax_pot + 1;
@@ -2120,7 +2172,7 @@ static int enqueue_build(eng_plan_t *P, hipStream_t st)
         hipLaunchKernelGGL(k_proj, dim3(P->n_proj), dim3(64), (size_t) P->proj_om * sizeof(double), sd, P->d_proj,
                            P->J, P->T, P->d_P);
     if (P->n_delta_centers > 0)
-        hipLaunchKernelGGL(k_delta, dim3(P->n_delta_centers), dim3(64), 0, sd, P->d_delta_centers,
+        hipLaunchKernelGGL(k_delta, dim3(P->n_delta_centers, P->delta_max_pairs), dim3(64), 0, sd, P->d_delta_centers,
                            P->n_delta_centers, P->J, P->d_delta);
     if (P->n_ocv > 0)   // one wave per item as well (the MMD sums are per thread)
         hipLaunchKernelGGL(k_ocv, dim3(P->n_ocv), dim3(64), (size_t) (P->ocv_scratch + 2 * P->ocv_nel) * sizeof(double),
